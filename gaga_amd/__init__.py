@@ -1,0 +1,170 @@
+"""gaga_amd: MI355X-native greedy sparse recovery.
+
+Host-side mirror of the reference's MATLAB interface for the recovery path:
+
+    gaga_cs(alg, ens, ...)   GAGA/gaga/gaga_cs.m:1-64      -> cs(alg, ens, ...)
+    gagaOptions(...)         GAGA/gaga/gagaOptions.m:61-124 -> options(...)
+
+The reference's licence (GAGA/GAGA_license:5) reserves the names gaga_cs /
+gaga_gen / gaga_smv / gaga_dct, so the entry point is called ``cs``; argument
+order, meaning and outputs follow gaga_cs.  Every call goes through the HIP
+library libgaga_amd.so (C ABI in include/gaga_amd.h); there is no CPU fallback:
+if the library or a GPU is missing, calls raise.
+"""
+import warnings
+
+import numpy as np
+
+from . import _capi
+from ._capi import Plan, GagaError, load  # noqa: F401
+
+__all__ = ["cs", "options", "Plan", "GagaError", "load", "ALGORITHMS", "ENSEMBLES"]
+
+ALGORITHMS = ("NIHT", "HTP", "CSMPSP", "CGIHT")
+ENSEMBLES = ("gen", "smv", "dct")
+
+# gagaOptions names (gagaOptions.m:8-58) -> handler
+_VEC_DIST = {"uniform": 0, "binary": 1, "gaussian": 2}
+_MAT_ENS = {"gen": {"gaussian": 1}, "smv": {"ones_scaled": 1, "binary": 2, "ones": 3}, "dct": {}}
+_ACCEPTED_IGNORED = {"threadsPerBlockn", "threadsPerBlockm", "threadsPerBlocknp",
+                     "threadsPerBlockBin", "kFixed", "alpha", "supportFlag", "timing",
+                     "projFracTol"}
+
+
+def options(*args, **kwargs):
+    """Build an options list like gagaOptions('tol', 1e-5, 'maxiter', 400).
+
+    Accepts alternating name/value positional pairs (the reference form) and/or
+    keyword arguments.  Returns a list of (name, value) pairs.
+    """
+    if len(args) % 2:
+        raise ValueError("options must come in name/value pairs")
+    opts = [(args[i], args[i + 1]) for i in range(0, len(args), 2)]
+    opts += list(kwargs.items())
+    for name, _ in opts:
+        if not isinstance(name, str):
+            raise ValueError("option names must be strings")
+    return opts
+
+
+def _apply_options(alg, ens, n, opts):
+    o = _capi.default_options(alg, ens, n)
+    for name, val in (opts or []):
+        if name == "tol":
+            o.tol = float(val)
+        elif name == "tolReferenceDefault":
+            o.tol_reference_default = int(bool(val))
+        elif name == "maxiter":
+            o.maxiter = int(val)
+        elif name == "numBins":
+            o.num_bins = int(val)
+        elif name == "restartFlag":
+            o.restart = 1 if val in ("on", 1, True) else 0
+        elif name == "seed":
+            o.seed = int(val) & 0xFFFFFFFF
+        elif name == "instance":
+            o.instance = int(val) & 0xFFFFFFFF
+        elif name == "vecDistribution":
+            if val not in _VEC_DIST:
+                raise ValueError("vecDistribution must be binary, uniform or gaussian")
+            o.vec_distribution = _VEC_DIST[val]
+        elif name == "matrixEnsemble":
+            table = _MAT_ENS[ens]
+            if val not in table:
+                raise ValueError("Admissible matrixEnsemble for %s are: %s" % (ens, ", ".join(table)))
+            o.matrix_ensemble = table[val]
+        elif name == "gpuNumber":
+            o.gpu_number = int(val)
+        elif name == "mode":
+            o.mode = {"device": 0, "plain": 1}[val] if isinstance(val, str) else int(val)
+        elif name == "teamSize":
+            o.team_size = int(val)
+        elif name == "blockSize":
+            o.block_size = int(val)
+        elif name == "convRateNum":
+            if int(val) != 16:
+                raise ValueError("convRateNum other than 16 is not supported")
+        elif name in _ACCEPTED_IGNORED:
+            continue
+        elif name == "noise":
+            if float(val) > 0:
+                raise ValueError("noisy problem generation is not part of this build")
+        else:
+            # gaga_gen.cu:315-317 prints and ignores unknown options
+            warnings.warn("The following option is not recognised: %s" % name)
+    return o
+
+
+def _check(alg, ens):
+    if ens not in ENSEMBLES:
+        raise ValueError("ens must be either dct, gen, or smv.")
+    if alg not in ALGORITHMS:
+        raise ValueError("The possible input strings for algorithms are: " + ", ".join(ALGORITHMS))
+
+
+def _is_array(a):
+    return np.ndim(a) > 0 and np.size(a) > 1
+
+
+def cs(alg, ens, *args):
+    """Mirror of gaga_cs (gaga_cs.m:7-64).
+
+    gen / dct, generate:  norms, times, iters, support, conv_rate, x = cs(alg, ens, k, m, n[, opts])
+    smv, generate:        norms, times, iters, support, conv_rate, x = cs(alg, 'smv', k, m, n, p[, opts])
+    gen, given problem:   x, iters, conv_rate = cs(alg, 'gen', y, A, k[, opts])
+    smv, given problem:   x, iters, conv_rate = cs(alg, 'smv', y, rows, cols, vals, k[, opts])
+    dct, given problem:   x, iters, conv_rate = cs(alg, 'dct', y, rows, k, m, n[, opts])
+
+    norms are the relative l1, l2, linf errors (results.cu:444-487); times are
+    [solve ms, ms per iteration, total ms]; support is [TP, FP, TN, FN].
+    Index arrays of the given-problem forms are 1-based like MATLAB's.
+    """
+    _check(alg, ens)
+    args = list(args)
+    opts = None
+    if args and isinstance(args[-1], (list, tuple)) and (not args[-1] or isinstance(args[-1][0], tuple)):
+        opts = args.pop()
+    if ens == "gen":
+        if len(args) >= 2 and _is_array(args[1]):
+            if len(args) != 3:
+                raise ValueError("Either five or six input arguments required.")
+            y, A, k = args
+            A = np.asarray(A, dtype=np.float32)
+            m, n = A.shape
+            o = _apply_options(alg, ens, n, opts)
+            return _capi.solve_gen(alg, np.asarray(y, np.float32), A, int(k), o)
+        if len(args) != 3:
+            raise ValueError("Either five or six input arguments required.")
+        k, m, n = (int(a) for a in args)
+        o = _apply_options(alg, ens, n, opts)
+        return _capi.generate(alg, ens, k, m, n, 0, o)
+    if ens == "smv":
+        if len(args) >= 3 and _is_array(args[2]):
+            if len(args) != 5:
+                raise ValueError("Either seven or eight input arguments required.")
+            y, rows, cols, vals, k = args
+            rows = np.asarray(rows, np.int32).ravel()
+            cols = np.asarray(cols, np.int32).ravel()
+            m = int(np.asarray(y).size)
+            n = int(cols.max()) if cols.size else 0
+            o = _apply_options(alg, ens, n, opts)
+            return _capi.solve_smv(alg, np.asarray(y, np.float32), rows, cols,
+                                   np.asarray(vals, np.float32).ravel(), m, n, int(k), o)
+        if len(args) != 4:
+            raise ValueError("Either six or seven input arguments required.")
+        k, m, n, p = (int(a) for a in args)
+        o = _apply_options(alg, ens, n, opts)
+        return _capi.generate(alg, ens, k, m, n, p, o)
+    # dct
+    if len(args) >= 2 and _is_array(args[1]):
+        if len(args) != 5:
+            raise ValueError("Either seven or eight input arguments required.")
+        y, rows, k, m, n = args
+        o = _apply_options(alg, ens, int(n), opts)
+        return _capi.solve_dct(alg, np.asarray(y, np.float32), np.asarray(rows, np.int32).ravel(),
+                               int(k), int(m), int(n), o)
+    if len(args) != 3:
+        raise ValueError("Either five or six input arguments required.")
+    k, m, n = (int(a) for a in args)
+    o = _apply_options(alg, ens, n, opts)
+    return _capi.generate(alg, ens, k, m, n, 0, o)

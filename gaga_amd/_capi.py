@@ -1,0 +1,217 @@
+"""ctypes binding of libgaga_amd.so (declarations in include/gaga_amd.h)."""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgaga_amd.so")
+
+# every symbol include/gaga_amd.h declares
+EXPORTS = (
+    "xcs_options_default", "xcs_cs_generate", "xcs_cs_gen", "xcs_cs_smv", "xcs_cs_dct",
+    "xcs_plan_create", "xcs_plan_solve", "xcs_plan_results", "xcs_plan_trace",
+    "xcs_plan_destroy", "xcs_find_support_set", "xcs_apply", "xcs_generate_gen",
+    "xcs_last_error", "xcs_version",
+)
+
+
+class GagaError(RuntimeError):
+    """A non-zero status from the C ABI."""
+
+    def __init__(self, code, msg):
+        super().__init__("xcs error %d: %s" % (code, msg))
+        self.code = code
+
+
+class Options(ctypes.Structure):
+    _fields_ = [
+        ("tol", ctypes.c_float), ("tol_reference_default", ctypes.c_int),
+        ("maxiter", ctypes.c_int), ("num_bins", ctypes.c_int), ("restart", ctypes.c_int),
+        ("seed", ctypes.c_uint), ("instance", ctypes.c_uint),
+        ("vec_distribution", ctypes.c_int), ("matrix_ensemble", ctypes.c_int),
+        ("gpu_number", ctypes.c_int), ("mode", ctypes.c_int), ("team_size", ctypes.c_int),
+        ("block_size", ctypes.c_int),
+    ]
+
+
+_lib = None
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_F = ctypes.c_float
+_C = ctypes.c_char_p
+
+
+def load():
+    """Load libgaga_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OSError("libgaga_amd.so not built: run `make -C gaga_amd` (or __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        L.xcs_options_default.argtypes = [_C, _C, _I, _P]
+        L.xcs_cs_generate.argtypes = [_C, _C, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P]
+        L.xcs_cs_gen.argtypes = [_C, _P, _P, _I, _I, _I, _P, _P, _P, _P]
+        L.xcs_cs_smv.argtypes = [_C, _P, _P, _P, _P, _I, _I, _I, _I, _P, _P, _P, _P]
+        L.xcs_cs_dct.argtypes = [_C, _P, _P, _I, _I, _I, _P, _P, _P, _P]
+        L.xcs_plan_create.argtypes = [_C, _C, _I, _I, _I, _I, _P, _P]
+        L.xcs_plan_solve.argtypes = [_P, _P]
+        L.xcs_plan_results.argtypes = [_P, _P, _P, _P, _P, _P, _P, _P]
+        L.xcs_plan_trace.argtypes = [_P, _P, _P, _I]
+        L.xcs_plan_destroy.argtypes = [_P]
+        L.xcs_plan_destroy.restype = None
+        L.xcs_find_support_set.argtypes = [_P, _I, _I, _I, _F, _F, _F, _P, _P, _P, _P, _P, _P]
+        L.xcs_apply.argtypes = [_C, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P]
+        L.xcs_generate_gen.argtypes = [_I, _I, ctypes.c_uint, ctypes.c_uint, _P]
+        L.xcs_last_error.restype = _C
+        L.xcs_version.restype = _C
+        _lib = L
+    return _lib
+
+
+def _ck(code):
+    if code != 0:
+        raise GagaError(code, load().xcs_last_error().decode())
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def default_options(alg, ens, n):
+    o = Options()
+    _ck(load().xcs_options_default(alg.encode(), ens.encode(), int(n), ctypes.byref(o)))
+    return o
+
+
+def generate(alg, ens, k, m, n, p, o):
+    norms = np.zeros(3, np.float32)
+    times = np.zeros(3, np.float32)
+    it = ctypes.c_int(0)
+    sup = np.zeros(4, np.int32)
+    rate = ctypes.c_float(0)
+    x = np.zeros(n, np.float32)
+    _ck(load().xcs_cs_generate(alg.encode(), ens.encode(), k, m, n, p, ctypes.byref(o), _p(norms),
+                               _p(times), ctypes.byref(it), _p(sup), ctypes.byref(rate), _p(x),
+                               None))
+    return norms, times, it.value, sup, rate.value, x
+
+
+def solve_gen(alg, y, A, k, o):
+    m, n = A.shape
+    Acm = np.asfortranarray(A, dtype=np.float32)  # column-major
+    x = np.zeros(n, np.float32)
+    it = ctypes.c_int(0)
+    rate = ctypes.c_float(0)
+    y = np.ascontiguousarray(y, np.float32)
+    _ck(load().xcs_cs_gen(alg.encode(), _p(y), Acm.ctypes.data_as(ctypes.c_void_p), m, n, k,
+                          ctypes.byref(o), _p(x), ctypes.byref(it), ctypes.byref(rate)))
+    return x, it.value, rate.value
+
+
+def solve_smv(alg, y, rows, cols, vals, m, n, k, o):
+    x = np.zeros(n, np.float32)
+    it = ctypes.c_int(0)
+    rate = ctypes.c_float(0)
+    rows = np.ascontiguousarray(rows, np.int32)
+    cols = np.ascontiguousarray(cols, np.int32)
+    vals = np.ascontiguousarray(vals, np.float32)
+    y = np.ascontiguousarray(y, np.float32)
+    _ck(load().xcs_cs_smv(alg.encode(), _p(y), _p(rows), _p(cols), _p(vals), rows.size, m, n, k,
+                          ctypes.byref(o), _p(x), ctypes.byref(it), ctypes.byref(rate)))
+    return x, it.value, rate.value
+
+
+def solve_dct(alg, y, rows, k, m, n, o):
+    x = np.zeros(n, np.float32)
+    it = ctypes.c_int(0)
+    rate = ctypes.c_float(0)
+    rows = np.ascontiguousarray(rows, np.int32)
+    y = np.ascontiguousarray(y, np.float32)
+    _ck(load().xcs_cs_dct(alg.encode(), _p(y), _p(rows), k, m, n, ctypes.byref(o), _p(x),
+                          ctypes.byref(it), ctypes.byref(rate)))
+    return x, it.value, rate.value
+
+
+def find_support_set(v, k, num_bins, slope, intercept, max_change, min_val, alpha, max_bin, k_bin):
+    """Device FindSupportSet; returns (sum, bins, state) like oracle.find_support_set."""
+    v = np.ascontiguousarray(v, np.float32)
+    bins = np.zeros(v.size, np.int32)
+    mv = ctypes.c_float(min_val)
+    al = ctypes.c_float(alpha)
+    mb = ctypes.c_int(max_bin)
+    kb = ctypes.c_int(k_bin)
+    s = ctypes.c_int(0)
+    _ck(load().xcs_find_support_set(_p(v), v.size, k, num_bins, slope, intercept, max_change,
+                                    ctypes.byref(mv), ctypes.byref(al), ctypes.byref(mb),
+                                    ctypes.byref(kb), _p(bins), ctypes.byref(s)))
+    return s.value, bins, dict(min_val=mv.value, alpha=al.value, max_bin=mb.value, k_bin=kb.value)
+
+
+def apply(ens, transpose, m, n, vec, A=None, col_ptr=None, rows=None, vals=None, dct_rows=None):
+    """y = A x (transpose=0) or g = A^T r (transpose=1) on the device."""
+    vec = np.ascontiguousarray(vec, np.float32)
+    out = np.zeros(n if transpose else m, np.float32)
+    Acm = None if A is None else np.asfortranarray(A, dtype=np.float32)
+    conv = lambda a, t: None if a is None else np.ascontiguousarray(a, t)  # noqa: E731
+    col_ptr, rows, vals, dct_rows = (conv(col_ptr, np.int32), conv(rows, np.int32),
+                                     conv(vals, np.float32), conv(dct_rows, np.int32))
+    _ck(load().xcs_apply(ens.encode(), int(transpose), m, n,
+                         None if Acm is None else Acm.ctypes.data_as(ctypes.c_void_p),
+                         _p(col_ptr), _p(rows), _p(vals), _p(dct_rows), _p(vec), _p(out)))
+    return out
+
+
+def generate_gen(m, n, seed, instance):
+    A = np.zeros((n, m), np.float32)  # row j = column j of A
+    _ck(load().xcs_generate_gen(m, n, seed, instance, _p(A)))
+    return A
+
+
+class Plan:
+    """A generated instance kept resident in HBM (xcs_plan_*)."""
+
+    def __init__(self, alg, ens, k, m, n, p=0, opts=None):
+        self.alg, self.ens, self.k, self.m, self.n, self.p = alg, ens, k, m, n, p
+        from . import _apply_options
+        o = _apply_options(alg, ens, n, opts)
+        self.maxiter = o.maxiter
+        h = ctypes.c_void_p()
+        _ck(load().xcs_plan_create(alg.encode(), ens.encode(), k, m, n, p, ctypes.byref(o),
+                                   ctypes.byref(h)))
+        self._h = h
+
+    def solve(self):
+        ms = ctypes.c_float(0)
+        _ck(load().xcs_plan_solve(self._h, ctypes.byref(ms)))
+        return ms.value
+
+    def results(self):
+        norms = np.zeros(3, np.float32)
+        sup = np.zeros(4, np.int32)
+        it = ctypes.c_int(0)
+        rate = ctypes.c_float(0)
+        reason = ctypes.c_int(0)
+        x = np.zeros(self.n, np.float32)
+        xt = np.zeros(self.n, np.float32)
+        _ck(load().xcs_plan_results(self._h, _p(norms), ctypes.byref(it), _p(sup),
+                                    ctypes.byref(rate), ctypes.byref(reason), _p(x), _p(xt)))
+        return dict(norms=norms, iters=it.value, support=sup, conv_rate=rate.value,
+                    stop_reason=reason.value, x=x, x_true=xt)
+
+    def trace(self, length):
+        kb = np.zeros(length, np.int32)
+        er = np.zeros(length, np.float32)
+        _ck(load().xcs_plan_trace(self._h, _p(kb), _p(er), length))
+        return kb, er
+
+    def close(self):
+        if self._h:
+            load().xcs_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,896 @@
+// Host side of the recovery engine: kernels, resident plans, problem
+// generation and the C ABI declared in include/gaga_amd.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/gaga_amd.h"
+#include "xcs_solver.h"
+
+using namespace xcs;
+
+// ------------------------------------------------------------------ errors --
+static thread_local std::string g_err;
+static int set_err(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+#define HIP_TRY(expr)                                                                  \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return set_err(e_ == hipErrorOutOfMemory ? XCS_ENOMEM : XCS_EHIP, "%s: %s (%s:%d)", \
+                     #expr, hipGetErrorString(e_), __FILE__, __LINE__);                \
+  } while (0)
+#define XCS_TRY(expr)        \
+  do {                       \
+    int r_ = (expr);         \
+    if (r_ != XCS_OK) return r_; \
+  } while (0)
+
+// ----------------------------------------------------------------- kernels --
+__global__ void __launch_bounds__(1024) solve_coop_kernel(const XProblem *P, int budget,
+                                                          double *red) {
+  __shared__ Lds L;
+  Team t{(int)blockIdx.x, (int)gridDim.x, red, 0};
+  solve_instance(t, L, *P, budget);
+}
+
+// Batched form: one workgroup per instance, instances taken from a queue.
+__global__ void __launch_bounds__(1024) solve_batch_kernel(const XProblem *Ps, int count,
+                                                           int *queue, int budget) {
+  __shared__ Lds L;
+  __shared__ int next;
+  Team t{0, 1, nullptr, 0};
+  while (true) {
+    if (threadIdx.x == 0) next = atomicAdd(queue, 1);
+    __syncthreads();
+    const int idx = next;
+    __syncthreads();
+    if (idx >= count) break;
+    solve_instance(t, L, Ps[idx], budget);
+    __syncthreads();
+  }
+}
+
+// y = A x_S (mode 0) or g = A^T r (mode 1) with the team operators.
+__global__ void __launch_bounds__(1024) apply_coop_kernel(const XProblem *P, int transpose,
+                                                          const float *in, float *out,
+                                                          int cnt, double *red) {
+  __shared__ Lds L;
+  Team t{(int)blockIdx.x, (int)gridDim.x, red, 0};
+  if (transpose) {
+    op_AT(t, L, *P, in, [&](int j, float g) { out[j] = g; });
+  } else {
+    op_A(t, L, *P, in, out, cnt);
+  }
+}
+
+// One FindSupportSet call with the team binning (functions.cu:397-436).
+__global__ void __launch_bounds__(1024) find_support_coop_kernel(const XProblem *P,
+                                                                 const float *v, XState *S_io,
+                                                                 float slope, float intercept,
+                                                                 double *red) {
+  __shared__ Lds L;
+  Team t{(int)blockIdx.x, (int)gridDim.x, red, 0};
+  XState S = *S_io;
+  if (S.min_val > S.max_change) {
+    if (t.rank == 0 && threadIdx.x == 0) {
+      S_io->min_val = S.min_val - S.max_change;
+      S_io->sum = -1;
+    }
+    return;
+  }
+  zero_counters(t, *P);
+  t.sync();
+  int kb;
+  find_support_bin(t, L, *P, v, P->bins, S, slope, intercept, &kb);
+  S.k_bin = kb;
+  if (t.rank == 0 && threadIdx.x == 0) *S_io = S;
+}
+
+// A[i + j*lda] ~ N(0,1)/sqrt(m) from Philox4x32-10, key (seed, instance),
+// counter (e mod 2^32, e div 2^32, STREAM_A, 0) with e = j*m + i; Box-Muller.
+__global__ void gen_matrix_kernel(float *A, int m, int n, int lda, uint32_t seed, uint32_t inst) {
+  const size_t total = (size_t)lda * n;
+  const double s = 1.0 / sqrt((double)m);
+  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (size_t)gridDim.x * blockDim.x) {
+    const int j = (int)(idx / lda), i = (int)(idx - (size_t)j * lda);
+    float v = 0.f;
+    if (i < m) {
+      const uint64_t e = (uint64_t)j * (uint64_t)m + (uint64_t)i;
+      uint32_t o[4];
+      philox4x32_10((uint32_t)e, (uint32_t)(e >> 32), STREAM_A, 0u, seed, inst, o);
+      const double z = sqrt(-2.0 * log(u01d(o[0]))) * cos(6.283185307179586476925286766559 * u01d(o[1]));
+      v = (float)(z * s);
+    }
+    A[idx] = v;
+  }
+}
+
+// DCT tables: tw[t] = exp(-2 pi i t / n), qw[q] = (cos, sin)(pi q / (2n)).
+__global__ void dct_tables_kernel(float2 *tw, int n) {
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+    double s, c;
+    sincospi(-2.0 * (double)t / (double)n, &s, &c);
+    tw[t] = make_float2((float)c, (float)s);
+    sincospi((double)t / (2.0 * (double)n), &s, &c);
+    tw[n + t] = make_float2((float)c, (float)s);
+  }
+}
+
+// ------------------------------------------------------------ host helpers --
+static int parse_alg(const char *alg) {
+  if (!alg) return -1;
+  if (!strcmp(alg, "NIHT")) return ALG_NIHT;
+  if (!strcmp(alg, "HTP")) return ALG_HTP;
+  if (!strcmp(alg, "CSMPSP")) return ALG_CSMPSP;
+  if (!strcmp(alg, "CGIHT")) return ALG_CGIHT;
+  return -1;
+}
+static int parse_ens(const char *ens) {
+  if (!ens) return -1;
+  if (!strcmp(ens, "gen")) return ENS_GEN;
+  if (!strcmp(ens, "smv")) return ENS_SMV;
+  if (!strcmp(ens, "dct")) return ENS_DCT;
+  return -1;
+}
+static int ilog2_exact(int n) {
+  if (n <= 0 || (n & (n - 1))) return -1;
+  int l = 0;
+  while ((1 << l) < n) l++;
+  return l;
+}
+static int round_up(int v, int a) { return (v + a - 1) / a * a; }
+
+// Host generators (Philox4x32-10, same streams as the device generator).
+static void host_draw(uint32_t seed, uint32_t inst, uint32_t a, uint32_t b, uint32_t stream,
+                      uint32_t o[4]) {
+  philox4x32_10(a, b, stream, 0u, seed, inst, o);
+}
+static void host_subset(int n, int k, uint32_t seed, uint32_t inst, uint32_t stream,
+                        std::vector<int> &idx) {
+  std::vector<unsigned char> sel((size_t)n, 0);
+  for (int d = 0; d < k; d++) {
+    const int j = n - k + d;
+    uint32_t o[4];
+    host_draw(seed, inst, (uint32_t)d, 0u, stream, o);
+    const int t = (int)bounded_u32(o[0], (uint32_t)(j + 1));
+    if (sel[t])
+      sel[j] = 1;
+    else
+      sel[t] = 1;
+  }
+  idx.clear();
+  for (int i = 0; i < n; i++)
+    if (sel[i]) idx.push_back(i);
+}
+static void host_sparse_vector(int n, int k, int dist, uint32_t seed, uint32_t inst,
+                               std::vector<float> &x, std::vector<int> &supp) {
+  host_subset(n, k, seed, inst, STREAM_SUPPORT, supp);
+  x.assign((size_t)n, 0.f);
+  for (int i : supp) {
+    uint32_t o[4];
+    host_draw(seed, inst, (uint32_t)i, 0u, STREAM_VALUES, o);
+    float v;
+    if (dist == 1)
+      v = (o[0] & 1u) ? 1.0f : -1.0f;
+    else if (dist == 2)
+      v = (float)(sqrt(-2.0 * log(u01d(o[0]))) * cos(6.283185307179586476925286766559 * u01d(o[1])));
+    else
+      v = (float)(((o[0] >> 8) + 0.5) * (1.0 / 16777216.0));
+    x[i] = v;
+  }
+}
+// smv: p distinct uniform rows per column by rejection, ascending in the column.
+static int host_smv(int m, int n, int p, int ens, uint32_t seed, uint32_t inst,
+                    std::vector<int> &rows, std::vector<float> &vals) {
+  if (p > m) return set_err(XCS_EINVAL, "smv: p (%d) > m (%d)", p, m);
+  rows.resize((size_t)n * p);
+  vals.resize((size_t)n * p);
+  const float scale = 1.0f / sqrtf((float)p);
+  for (int j = 0; j < n; j++) {
+    int *r = rows.data() + (size_t)j * p;
+    int got = 0;
+    for (uint32_t t = 0; got < p; t++) {
+      uint32_t o[4];
+      host_draw(seed, inst, (uint32_t)j, t, STREAM_SMV_ROWS, o);
+      const int cand = (int)bounded_u32(o[0], (uint32_t)m);
+      bool dup = false;
+      for (int q = 0; q < got; q++) dup |= (r[q] == cand);
+      if (!dup) r[got++] = cand;
+    }
+    std::sort(r, r + p);
+    for (int t = 0; t < p; t++) {
+      float v = 1.0f;
+      if (ens == 2) {
+        uint32_t o[4];
+        host_draw(seed, inst, (uint32_t)j, (uint32_t)t, STREAM_SMV_VALS, o);
+        v = (o[0] & 1u) ? 1.0f : -1.0f;
+      }
+      if (ens == 1 || ens == 2) v = v * scale;
+      vals[(size_t)j * p + t] = v;
+    }
+  }
+  return XCS_OK;
+}
+
+// ------------------------------------------------------------------- plans --
+struct xcs_plan {
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  XProblem h{};         // host copy (device pointers)
+  XProblem *d = nullptr;
+  int team = 1, block = 256, mode = 0;
+  double *red = nullptr;
+  std::vector<void *> allocs;
+  std::vector<float> x_true;
+  int k_planted = 0;
+  float gen_ms = 0.f, last_ms = 0.f;
+  XState last{};
+  ~xcs_plan() {
+    for (void *p : allocs) (void)hipFree(p);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+  template <class T>
+  int alloc(T **p, size_t count) {
+    void *q = nullptr;
+    const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    hipError_t e = hipMalloc(&q, bytes);
+    if (e != hipSuccess)
+      return set_err(XCS_ENOMEM, "hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(e));
+    allocs.push_back(q);
+    e = hipMemsetAsync(q, 0, bytes, stream);
+    if (e != hipSuccess) return set_err(XCS_EHIP, "hipMemsetAsync: %s", hipGetErrorString(e));
+    *p = (T *)q;
+    return XCS_OK;
+  }
+};
+
+static int max_coop_blocks(int block, int *out) {
+  int dev;
+  HIP_TRY(hipGetDevice(&dev));
+  int cus = 0;
+  HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  // every cooperative kernel must fit co-resident at the chosen grid size
+  const void *kernels[] = {(const void *)solve_coop_kernel, (const void *)apply_coop_kernel,
+                           (const void *)find_support_coop_kernel};
+  int per = 1 << 30;
+  for (const void *k : kernels) {
+    int b = 0;
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k, block, 0));
+    per = std::min(per, b);
+  }
+  if (per <= 0) return set_err(XCS_EINVAL, "block size %d does not fit on a CU", block);
+  *out = per * cus;
+  return XCS_OK;
+}
+
+static int use_device(int gpu) {
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count <= 0) return set_err(XCS_ENODEV, "no HIP device available");
+  if (gpu < 0 || gpu >= count) gpu = 0;  // gaga_gen.cu:248-253 resets silently
+  HIP_TRY(hipSetDevice(gpu));
+  return XCS_OK;
+}
+
+// Allocate workspace for (alg, ens, m, n) and fill the sizes in plan->h.
+static int plan_alloc(xcs_plan *pl, int alg, int ens, int k, int m, int n, int p, int nz,
+                      const xcs_options *o) {
+  XProblem &P = pl->h;
+  P.alg = alg;
+  P.ens = ens;
+  P.m = m;
+  P.n = n;
+  P.k = k;
+  P.p = p;
+  P.nz = nz;
+  P.lda = round_up(m, 64);
+  P.num_bins = o->num_bins > 0 ? o->num_bins : std::max(n / 20, 1000);
+  P.maxiter = o->maxiter > 0 ? o->maxiter : ((alg == ALG_HTP || alg == ALG_CSMPSP) ? 300 : 5000);
+  P.restart = o->restart;
+  P.tol = o->tol_reference_default ? (float)(10 ^ (-4)) : o->tol;
+  P.log2n = ens == ENS_DCT ? ilog2_exact(n) : 0;
+  pl->mode = o->mode;
+  pl->block = o->block_size > 0 ? o->block_size : 256;
+  if (pl->block % 64 || pl->block > 1024) return set_err(XCS_EINVAL, "block_size must be a multiple of 64 <= 1024");
+  int maxb;
+  XCS_TRY(max_coop_blocks(pl->block, &maxb));
+  int team = o->team_size;
+  if (team <= 0) {
+    const double bytes = ens == ENS_GEN ? 4.0 * P.lda * (double)n
+                         : ens == ENS_SMV ? 8.0 * nz : 24.0 * n;
+    team = (int)std::min<double>(maxb, std::max(1.0, bytes / (256.0 * 1024.0)));
+  }
+  pl->team = std::min(team, maxb);
+  const size_t nn = (size_t)n, ml = (size_t)P.lda;
+  XCS_TRY(pl->alloc(&P.x, nn));
+  XCS_TRY(pl->alloc(&P.thres, nn));
+  XCS_TRY(pl->alloc(&P.grad, nn));
+  XCS_TRY(pl->alloc(&P.gprev, nn));
+  XCS_TRY(pl->alloc(&P.gprev_thres, nn));
+  XCS_TRY(pl->alloc(&P.bins, nn));
+  XCS_TRY(pl->alloc(&P.bins2, nn));
+  XCS_TRY(pl->alloc(&P.resid, ml));
+  XCS_TRY(pl->alloc(&P.resid_update, ml));
+  XCS_TRY(pl->alloc(&P.resid2, ml));
+  XCS_TRY(pl->alloc(&P.counters, (size_t)P.num_bins + 64));
+  XCS_TRY(pl->alloc(&P.supp, nn));
+  XCS_TRY(pl->alloc(&P.wg_counts, (size_t)pl->team + 64));
+  if (ens == ENS_GEN) XCS_TRY(pl->alloc(&P.partials, (size_t)MAXCH * ml));
+  if (ens == ENS_DCT) {
+    XCS_TRY(pl->alloc(&P.cwork, 3 * nn));
+    float2 *tw;
+    XCS_TRY(pl->alloc(&tw, 2 * nn));
+    hipLaunchKernelGGL(dct_tables_kernel, dim3(std::min(4096, (n + 255) / 256)), dim3(256), 0,
+                       pl->stream, tw, n);
+    HIP_TRY(hipGetLastError());
+    P.twiddle = tw;
+  }
+  XCS_TRY(pl->alloc(&P.st, 1));
+  XCS_TRY(pl->alloc(&P.trace_err, (size_t)P.maxiter + 2));
+  XCS_TRY(pl->alloc(&P.trace_kbin, (size_t)P.maxiter + 2));
+  XCS_TRY(pl->alloc(&pl->red, (size_t)2 * RED_SLOTS * (pl->team + 1)));
+  XCS_TRY(pl->alloc(&pl->d, 1));
+  return XCS_OK;
+}
+
+static int plan_upload_desc(xcs_plan *pl) {
+  HIP_TRY(hipMemcpyAsync(pl->d, &pl->h, sizeof(XProblem), hipMemcpyHostToDevice, pl->stream));
+  return XCS_OK;
+}
+
+static int launch_apply(xcs_plan *pl, int transpose, const float *in, float *out, int cnt) {
+  void *args[] = {(void *)&pl->d, (void *)&transpose, (void *)&in, (void *)&out, (void *)&cnt,
+                  (void *)&pl->red};
+  HIP_TRY(hipLaunchCooperativeKernel((const void *)apply_coop_kernel, dim3(pl->team),
+                                     dim3(pl->block), args, 0, pl->stream));
+  return XCS_OK;
+}
+
+// Upload a dense x with its support list into P.supp, compute y = A x.
+static int plan_make_y(xcs_plan *pl, const std::vector<float> &x, const std::vector<int> &supp,
+                       float *d_x, float *d_y) {
+  XProblem &P = pl->h;
+  HIP_TRY(hipMemcpyAsync(d_x, x.data(), sizeof(float) * x.size(), hipMemcpyHostToDevice, pl->stream));
+  if (!supp.empty())
+    HIP_TRY(hipMemcpyAsync(P.supp, supp.data(), sizeof(int) * supp.size(), hipMemcpyHostToDevice,
+                           pl->stream));
+  XCS_TRY(plan_upload_desc(pl));
+  XCS_TRY(launch_apply(pl, 0, d_x, d_y, (int)supp.size()));
+  HIP_TRY(hipStreamSynchronize(pl->stream));
+  return XCS_OK;
+}
+
+// Build the CSC (fixed or general) and CSR device copies of an smv matrix.
+static int plan_set_smv(xcs_plan *pl, const std::vector<int> &col_ptr, const std::vector<int> &rows,
+                        const std::vector<float> &vals) {
+  XProblem &P = pl->h;
+  const int m = P.m, n = P.n, nz = (int)rows.size();
+  std::vector<int> row_ptr((size_t)m + 1, 0), cols((size_t)nz);
+  std::vector<float> rvals((size_t)nz);
+  for (int e = 0; e < nz; e++) row_ptr[rows[e] + 1]++;
+  for (int i = 0; i < m; i++) row_ptr[i + 1] += row_ptr[i];
+  std::vector<int> fill(row_ptr.begin(), row_ptr.end() - 1);
+  for (int j = 0; j < n; j++)
+    for (int e = col_ptr[j]; e < col_ptr[j + 1]; e++) {
+      const int pos = fill[rows[e]]++;
+      cols[pos] = j;
+      rvals[pos] = vals[e];
+    }
+  int *d_cp, *d_r, *d_rp, *d_c;
+  float *d_v, *d_rv;
+  XCS_TRY(pl->alloc(&d_cp, (size_t)n + 1));
+  XCS_TRY(pl->alloc(&d_r, (size_t)nz));
+  XCS_TRY(pl->alloc(&d_v, (size_t)nz));
+  XCS_TRY(pl->alloc(&d_rp, (size_t)m + 1));
+  XCS_TRY(pl->alloc(&d_c, (size_t)nz));
+  XCS_TRY(pl->alloc(&d_rv, (size_t)nz));
+  HIP_TRY(hipMemcpyAsync(d_cp, col_ptr.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, pl->stream));
+  HIP_TRY(hipMemcpyAsync(d_r, rows.data(), sizeof(int) * nz, hipMemcpyHostToDevice, pl->stream));
+  HIP_TRY(hipMemcpyAsync(d_v, vals.data(), sizeof(float) * nz, hipMemcpyHostToDevice, pl->stream));
+  HIP_TRY(hipMemcpyAsync(d_rp, row_ptr.data(), sizeof(int) * (m + 1), hipMemcpyHostToDevice, pl->stream));
+  HIP_TRY(hipMemcpyAsync(d_c, cols.data(), sizeof(int) * nz, hipMemcpyHostToDevice, pl->stream));
+  HIP_TRY(hipMemcpyAsync(d_rv, rvals.data(), sizeof(float) * nz, hipMemcpyHostToDevice, pl->stream));
+  HIP_TRY(hipStreamSynchronize(pl->stream));
+  P.col_ptr = d_cp;
+  P.rows = d_r;
+  P.vals = d_v;
+  P.row_ptr = d_rp;
+  P.cols = d_c;
+  P.rvals = d_rv;
+  P.nz = nz;
+  return XCS_OK;
+}
+
+static int validate(int alg, int ens, int k, int m, int n) {
+  if (alg < 0) return set_err(XCS_EINVAL, "algorithm must be one of NIHT, HTP, CSMPSP, CGIHT");
+  if (ens < 0) return set_err(XCS_EINVAL, "ens must be either dct, gen, or smv.");
+  if (m <= 0 || n <= 0 || k < 0) return set_err(XCS_EINVAL, "need m > 0, n > 0, k >= 0");
+  if (ens == ENS_DCT && (ilog2_exact(n) < 1 || n > (1 << 22)))
+    return set_err(XCS_EINVAL, "dct: n must be a power of two in [2, 2^22]");
+  if (ens == ENS_DCT && m > n) return set_err(XCS_EINVAL, "dct: m must be <= n");
+  return XCS_OK;
+}
+
+// Generate an instance into a fresh plan (seed, instance from opts).
+static int plan_generate(xcs_plan *pl, int alg, int ens, int k, int m, int n, int p,
+                         const xcs_options *o) {
+  XCS_TRY(validate(alg, ens, k, m, n));
+  if (k > n) return set_err(XCS_EINVAL, "k must be <= n");
+  XCS_TRY(use_device(o->gpu_number));
+  HIP_TRY(hipGetDevice(&pl->dev));
+  HIP_TRY(hipStreamCreateWithFlags(&pl->stream, hipStreamNonBlocking));
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  HIP_TRY(hipEventRecord(e0, pl->stream));
+  const uint32_t seed = o->seed, inst = o->instance;
+  std::vector<int> csc_rows, csc_ptr;
+  std::vector<float> csc_vals;
+  int nz = 0;
+  if (ens == ENS_SMV) {
+    if (p <= 0) return set_err(XCS_EINVAL, "smv: p must be > 0");
+    XCS_TRY(host_smv(m, n, p, o->matrix_ensemble, seed, inst, csc_rows, csc_vals));
+    nz = n * p;
+  }
+  XCS_TRY(plan_alloc(pl, alg, ens, k, m, n, p, nz, o));
+  XProblem &P = pl->h;
+  float *y;
+  XCS_TRY(pl->alloc(&y, (size_t)P.lda));
+  P.y = y;
+  if (ens == ENS_GEN) {
+    float *A;
+    XCS_TRY(pl->alloc(&A, (size_t)P.lda * n));
+    const size_t total = (size_t)P.lda * n;
+    const int blocks = (int)std::min<size_t>(65536, (total + 255) / 256);
+    hipLaunchKernelGGL(gen_matrix_kernel, dim3(blocks), dim3(256), 0, pl->stream, A, m, n, P.lda,
+                       seed, inst);
+    HIP_TRY(hipGetLastError());
+    P.A = A;
+  } else if (ens == ENS_SMV) {
+    csc_ptr.resize((size_t)n + 1);
+    for (int j = 0; j <= n; j++) csc_ptr[j] = j * p;
+    XCS_TRY(plan_set_smv(pl, csc_ptr, csc_rows, csc_vals));
+  } else {
+    std::vector<int> rows;
+    host_subset(n, m, seed, inst, STREAM_DCT_ROWS, rows);
+    int *d_rows;
+    XCS_TRY(pl->alloc(&d_rows, (size_t)m));
+    HIP_TRY(hipMemcpyAsync(d_rows, rows.data(), sizeof(int) * m, hipMemcpyHostToDevice, pl->stream));
+    P.dct_rows = d_rows;
+  }
+  std::vector<int> supp;
+  host_sparse_vector(n, k, o->vec_distribution, seed, inst, pl->x_true, supp);
+  pl->k_planted = k;
+  float *d_xt;
+  XCS_TRY(pl->alloc(&d_xt, (size_t)n));
+  XCS_TRY(plan_make_y(pl, pl->x_true, supp, d_xt, y));
+  HIP_TRY(hipEventRecord(e1, pl->stream));
+  HIP_TRY(hipEventSynchronize(e1));
+  HIP_TRY(hipEventElapsedTime(&pl->gen_ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return XCS_OK;
+}
+
+static int plan_run(xcs_plan *pl, float *ms) {
+  XProblem &P = pl->h;
+  // reset: x = 0 and a fresh state (inputs stay resident)
+  HIP_TRY(hipMemsetAsync(P.x, 0, sizeof(float) * P.n, pl->stream));
+  HIP_TRY(hipMemsetAsync(P.st, 0, sizeof(XState), pl->stream));
+  XCS_TRY(plan_upload_desc(pl));
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  HIP_TRY(hipEventRecord(e0, pl->stream));
+  if (pl->mode == 1) {
+    // plain form: one outer iteration per launch, host reads the stop flag
+    int budget = 1;
+    for (int launch = 0; launch < P.maxiter + 4; launch++) {
+      void *args[] = {(void *)&pl->d, (void *)&budget, (void *)&pl->red};
+      HIP_TRY(hipLaunchCooperativeKernel((const void *)solve_coop_kernel, dim3(pl->team),
+                                         dim3(pl->block), args, 0, pl->stream));
+      int done = 0;
+      HIP_TRY(hipMemcpyAsync(&done, &P.st->done, sizeof(int), hipMemcpyDeviceToHost, pl->stream));
+      HIP_TRY(hipStreamSynchronize(pl->stream));
+      if (done) break;
+    }
+  } else {
+    int budget = P.maxiter + 4;
+    void *args[] = {(void *)&pl->d, (void *)&budget, (void *)&pl->red};
+    HIP_TRY(hipLaunchCooperativeKernel((const void *)solve_coop_kernel, dim3(pl->team),
+                                       dim3(pl->block), args, 0, pl->stream));
+  }
+  HIP_TRY(hipEventRecord(e1, pl->stream));
+  HIP_TRY(hipEventSynchronize(e1));
+  HIP_TRY(hipEventElapsedTime(&pl->last_ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  HIP_TRY(hipMemcpy(&pl->last, P.st, sizeof(XState), hipMemcpyDeviceToHost));
+  if (ms) *ms = pl->last_ms;
+  return XCS_OK;
+}
+
+// conv rate forms: results.cu:365-378 (generated) and gaga_gen.cu:735-742 (given)
+static float conv_rate_results(const XState &S) {
+  int temp = std::min(S.iter, 16);
+  if (temp <= 1) return 0.0f;
+  temp = temp - 1;
+  const float root = 1 / (float)temp;
+  temp = 15 - temp;
+  return powf(S.prev[15] / S.prev[temp], root);
+}
+static float conv_rate_given(const XState &S) {
+  int temp = std::min(S.iter, 16);
+  if (temp == 0) return 0.0f;
+  const float root = 1 / (float)temp;
+  temp = 16 - temp;
+  return powf(S.prev[15] / S.prev[temp], root);
+}
+
+static void host_norms(const std::vector<float> &xt, const float *xo, int k, float norms[3],
+                       int support[4]) {
+  double a1 = 0, a2 = 0, ai = 0, d1 = 0, d2 = 0, di = 0;
+  int tp = 0, tn = 0;
+  const int n = (int)xt.size();
+  for (int i = 0; i < n; i++) {
+    const double t = xt[i], o = xo[i], df = t - o;
+    a1 += fabs(t);
+    a2 += t * t;
+    ai = std::max(ai, fabs(t));
+    d1 += fabs(df);
+    d2 += df * df;
+    di = std::max(di, fabs(df));
+    if (t != 0.0) {
+      if (o != 0.0) tp++;
+    } else if (o == 0.0) {
+      tn++;
+    }
+  }
+  if (norms) {
+    norms[0] = (float)(d1 / a1);
+    norms[1] = (float)(sqrt(d2) / sqrt(a2));
+    norms[2] = (float)(di / ai);
+  }
+  if (support) {
+    support[0] = tp;
+    support[1] = k - tp;
+    support[2] = tn;
+    support[3] = n - k - tn;
+  }
+}
+
+// ------------------------------------------------------------------ C ABI --
+extern "C" {
+
+const char *xcs_last_error(void) { return g_err.c_str(); }
+const char *xcs_version(void) { return "gaga_amd 0.1 (gfx950)"; }
+
+int xcs_options_default(const char *alg, const char *ens, int n, xcs_options *o) {
+  if (!o) return set_err(XCS_EINVAL, "opts is NULL");
+  const int a = parse_alg(alg), e = parse_ens(ens);
+  if (a < 0) return set_err(XCS_EINVAL, "unknown algorithm '%s'", alg ? alg : "(null)");
+  if (e < 0) return set_err(XCS_EINVAL, "ens must be either dct, gen, or smv.");
+  memset(o, 0, sizeof(*o));
+  o->tol = 1e-4f;  // the documented intent (gagaOptions.m); see tol_reference_default
+  o->tol_reference_default = 0;
+  o->maxiter = (a == ALG_HTP || a == ALG_CSMPSP) ? 300 : 5000;  // gaga_gen.cu:191-192
+  o->num_bins = std::max(n / 20, 1000);                       // gaga_gen.cu:180
+  o->restart = 0;
+  o->seed = 0;
+  o->instance = 0;
+  o->vec_distribution = 1;                       // binary
+  o->matrix_ensemble = e == ENS_SMV ? 3 : 1;     // smv default 'ones' (gaga_smv.cu:241)
+  o->gpu_number = 0;
+  o->mode = 0;
+  o->team_size = 0;
+  o->block_size = 0;
+  return XCS_OK;
+}
+
+int xcs_plan_create(const char *alg, const char *ens, int k, int m, int n, int p,
+                    const xcs_options *opts, xcs_plan **out) {
+  if (!out || !opts) return set_err(XCS_EINVAL, "NULL argument");
+  *out = nullptr;
+  xcs_plan *pl = new xcs_plan();
+  int r = plan_generate(pl, parse_alg(alg), parse_ens(ens), k, m, n, p, opts);
+  if (r != XCS_OK) {
+    delete pl;
+    return r;
+  }
+  *out = pl;
+  return XCS_OK;
+}
+
+int xcs_plan_solve(xcs_plan *pl, float *ms) {
+  if (!pl) return set_err(XCS_EINVAL, "NULL plan");
+  HIP_TRY(hipSetDevice(pl->dev));
+  return plan_run(pl, ms);
+}
+
+int xcs_plan_results(xcs_plan *pl, float norms[3], int *iterations, int support[4],
+                     float *conv_rate, int *stop_reason, float *x_out, float *x_true) {
+  if (!pl) return set_err(XCS_EINVAL, "NULL plan");
+  HIP_TRY(hipSetDevice(pl->dev));
+  const int n = pl->h.n;
+  std::vector<float> xo((size_t)n);
+  HIP_TRY(hipMemcpy(xo.data(), pl->h.x, sizeof(float) * n, hipMemcpyDeviceToHost));
+  if (!pl->x_true.empty()) host_norms(pl->x_true, xo.data(), pl->k_planted, norms, support);
+  if (iterations) *iterations = pl->last.iter;
+  if (conv_rate) *conv_rate = conv_rate_results(pl->last);
+  if (stop_reason) *stop_reason = pl->last.stop_reason;
+  if (x_out) memcpy(x_out, xo.data(), sizeof(float) * n);
+  if (x_true && !pl->x_true.empty()) memcpy(x_true, pl->x_true.data(), sizeof(float) * n);
+  return XCS_OK;
+}
+
+int xcs_plan_trace(xcs_plan *pl, int *k_bin, float *resid_norm, int len) {
+  if (!pl) return set_err(XCS_EINVAL, "NULL plan");
+  HIP_TRY(hipSetDevice(pl->dev));
+  len = std::min(len, pl->h.maxiter + 2);
+  if (k_bin) HIP_TRY(hipMemcpy(k_bin, pl->h.trace_kbin, sizeof(int) * len, hipMemcpyDeviceToHost));
+  if (resid_norm)
+    HIP_TRY(hipMemcpy(resid_norm, pl->h.trace_err, sizeof(float) * len, hipMemcpyDeviceToHost));
+  return XCS_OK;
+}
+
+void xcs_plan_destroy(xcs_plan *pl) { delete pl; }
+
+int xcs_cs_generate(const char *alg, const char *ens, int k, int m, int n, int p,
+                    const xcs_options *opts, float norms[3], float times[3], int *iterations,
+                    int support[4], float *conv_rate, float *x_out, float *x_true) {
+  xcs_options o;
+  if (opts)
+    o = *opts;
+  else
+    XCS_TRY(xcs_options_default(alg, ens, n, &o));
+  xcs_plan *pl = nullptr;
+  XCS_TRY(xcs_plan_create(alg, ens, k, m, n, p, &o, &pl));
+  float ms = 0.f;
+  int r = xcs_plan_solve(pl, &ms);
+  if (r == XCS_OK)
+    r = xcs_plan_results(pl, norms, iterations, support, conv_rate, nullptr, x_out, x_true);
+  if (r == XCS_OK && times) {
+    times[0] = ms;
+    times[1] = pl->last.iter > 0 ? ms / (float)pl->last.iter : 0.f;
+    times[2] = ms + pl->gen_ms;
+  }
+  xcs_plan_destroy(pl);
+  return r;
+}
+
+// Shared tail of the solve-given-problem forms.
+static int given_solve(xcs_plan *pl, const float *y, float *x_out, int *iterations,
+                       float *conv_rate) {
+  XProblem &P = pl->h;
+  float *d_y;
+  XCS_TRY(pl->alloc(&d_y, (size_t)P.lda));
+  HIP_TRY(hipMemcpyAsync(d_y, y, sizeof(float) * P.m, hipMemcpyHostToDevice, pl->stream));
+  P.y = d_y;
+  XCS_TRY(plan_run(pl, nullptr));
+  if (x_out) HIP_TRY(hipMemcpy(x_out, P.x, sizeof(float) * P.n, hipMemcpyDeviceToHost));
+  if (iterations) *iterations = pl->last.iter;
+  if (conv_rate) *conv_rate = conv_rate_given(pl->last);
+  return XCS_OK;
+}
+
+static int given_begin(xcs_plan *pl, int gpu) {
+  XCS_TRY(use_device(gpu));
+  HIP_TRY(hipGetDevice(&pl->dev));
+  HIP_TRY(hipStreamCreateWithFlags(&pl->stream, hipStreamNonBlocking));
+  return XCS_OK;
+}
+
+int xcs_cs_gen(const char *alg, const float *y, const float *A, int m, int n, int k,
+               const xcs_options *opts, float *x_out, int *iterations, float *conv_rate) {
+  const int a = parse_alg(alg);
+  XCS_TRY(validate(a, ENS_GEN, k, m, n));
+  if (!y || !A) return set_err(XCS_EINVAL, "NULL y or A");
+  xcs_options o;
+  if (opts)
+    o = *opts;
+  else
+    XCS_TRY(xcs_options_default(alg, "gen", n, &o));
+  xcs_plan pl;
+  XCS_TRY(given_begin(&pl, o.gpu_number));
+  XCS_TRY(plan_alloc(&pl, a, ENS_GEN, k, m, n, 0, 0, &o));
+  float *dA;
+  XCS_TRY(pl.alloc(&dA, (size_t)pl.h.lda * n));
+  HIP_TRY(hipMemcpy2DAsync(dA, sizeof(float) * pl.h.lda, A, sizeof(float) * m, sizeof(float) * m, n,
+                           hipMemcpyHostToDevice, pl.stream));
+  pl.h.A = dA;
+  return given_solve(&pl, y, x_out, iterations, conv_rate);
+}
+
+int xcs_cs_smv(const char *alg, const float *y, const int *rows, const int *cols,
+               const float *vals, int nz, int m, int n, int k, const xcs_options *opts,
+               float *x_out, int *iterations, float *conv_rate) {
+  const int a = parse_alg(alg);
+  XCS_TRY(validate(a, ENS_SMV, k, m, n));
+  if (!y || !rows || !cols || !vals || nz <= 0) return set_err(XCS_EINVAL, "bad smv arrays");
+  // 1-based triplets -> 0-based CSC sorted by (col, row)
+  std::vector<int> cp((size_t)n + 1, 0), rr((size_t)nz);
+  std::vector<float> vv((size_t)nz);
+  for (int e = 0; e < nz; e++) {
+    const int r = rows[e] - 1, c = cols[e] - 1;
+    if (r < 0 || r >= m || c < 0 || c >= n)
+      return set_err(XCS_EINVAL, "smv entry %d out of range (row %d, col %d)", e, rows[e], cols[e]);
+    cp[c + 1]++;
+  }
+  for (int j = 0; j < n; j++) cp[j + 1] += cp[j];
+  std::vector<int> order((size_t)nz);
+  for (int e = 0; e < nz; e++) order[e] = e;
+  std::stable_sort(order.begin(), order.end(), [&](int a_, int b_) {
+    if (cols[a_] != cols[b_]) return cols[a_] < cols[b_];
+    return rows[a_] < rows[b_];
+  });
+  for (int q = 0; q < nz; q++) {
+    rr[q] = rows[order[q]] - 1;
+    vv[q] = vals[order[q]];
+  }
+  xcs_options o;
+  if (opts)
+    o = *opts;
+  else
+    XCS_TRY(xcs_options_default(alg, "smv", n, &o));
+  xcs_plan pl;
+  XCS_TRY(given_begin(&pl, o.gpu_number));
+  XCS_TRY(plan_alloc(&pl, a, ENS_SMV, k, m, n, 0, nz, &o));
+  XCS_TRY(plan_set_smv(&pl, cp, rr, vv));
+  return given_solve(&pl, y, x_out, iterations, conv_rate);
+}
+
+int xcs_cs_dct(const char *alg, const float *y, const int *rows, int k, int m, int n,
+               const xcs_options *opts, float *x_out, int *iterations, float *conv_rate) {
+  const int a = parse_alg(alg);
+  XCS_TRY(validate(a, ENS_DCT, k, m, n));
+  if (!y || !rows) return set_err(XCS_EINVAL, "NULL y or rows");
+  std::vector<int> r0((size_t)m);
+  for (int i = 0; i < m; i++) {
+    r0[i] = rows[i] - 1;
+    if (r0[i] < 0 || r0[i] >= n) return set_err(XCS_EINVAL, "dct row %d out of range", rows[i]);
+  }
+  xcs_options o;
+  if (opts)
+    o = *opts;
+  else
+    XCS_TRY(xcs_options_default(alg, "dct", n, &o));
+  xcs_plan pl;
+  XCS_TRY(given_begin(&pl, o.gpu_number));
+  XCS_TRY(plan_alloc(&pl, a, ENS_DCT, k, m, n, 0, 0, &o));
+  int *d_rows;
+  XCS_TRY(pl.alloc(&d_rows, (size_t)m));
+  HIP_TRY(hipMemcpyAsync(d_rows, r0.data(), sizeof(int) * m, hipMemcpyHostToDevice, pl.stream));
+  pl.h.dct_rows = d_rows;
+  return given_solve(&pl, y, x_out, iterations, conv_rate);
+}
+
+int xcs_find_support_set(const float *v, int n, int k, int num_bins, float slope, float intercept,
+                         float max_change, float *min_val, float *alpha, int *max_bin, int *k_bin,
+                         int *bins, int *sum) {
+  if (!v || !min_val || !alpha || !max_bin || !k_bin || !bins || !sum || n <= 0 || num_bins <= 0)
+    return set_err(XCS_EINVAL, "bad arguments");
+  if (*max_bin > num_bins) return set_err(XCS_EINVAL, "max_bin > num_bins");
+  xcs_options o;
+  XCS_TRY(xcs_options_default("NIHT", "gen", n, &o));
+  o.num_bins = num_bins;
+  xcs_plan pl;
+  XCS_TRY(given_begin(&pl, 0));
+  XCS_TRY(plan_alloc(&pl, ALG_NIHT, ENS_GEN, k, 1, n, 0, 0, &o));
+  float *dv;
+  XCS_TRY(pl.alloc(&dv, (size_t)n));
+  HIP_TRY(hipMemcpyAsync(dv, v, sizeof(float) * n, hipMemcpyHostToDevice, pl.stream));
+  XState S{};
+  S.min_val = *min_val;
+  S.max_change = max_change;
+  S.alpha_bin = *alpha;
+  S.max_bin = *max_bin;
+  S.k_bin = *k_bin;
+  HIP_TRY(hipMemcpyAsync(pl.h.st, &S, sizeof(S), hipMemcpyHostToDevice, pl.stream));
+  XCS_TRY(plan_upload_desc(&pl));
+  void *args[] = {(void *)&pl.d, (void *)&dv, (void *)&pl.h.st, (void *)&slope,
+                  (void *)&intercept, (void *)&pl.red};
+  HIP_TRY(hipLaunchCooperativeKernel((const void *)find_support_coop_kernel, dim3(pl.team),
+                                     dim3(pl.block), args, 0, pl.stream));
+  HIP_TRY(hipStreamSynchronize(pl.stream));
+  HIP_TRY(hipMemcpy(&S, pl.h.st, sizeof(S), hipMemcpyDeviceToHost));
+  *min_val = S.min_val;
+  *alpha = S.alpha_bin;
+  *max_bin = S.max_bin;
+  *k_bin = S.k_bin;
+  *sum = S.sum;
+  if (S.sum >= 0) HIP_TRY(hipMemcpy(bins, pl.h.bins, sizeof(int) * n, hipMemcpyDeviceToHost));
+  return XCS_OK;
+}
+
+int xcs_apply(const char *ens, int transpose, int m, int n, const float *A, const int *col_ptr,
+              const int *rows, const float *vals, const int *dct_rows, const float *in,
+              float *out) {
+  const int e = parse_ens(ens);
+  XCS_TRY(validate(ALG_NIHT, e, 0, m, n));
+  if (!in || !out) return set_err(XCS_EINVAL, "NULL in/out");
+  xcs_options o;
+  XCS_TRY(xcs_options_default("NIHT", ens, n, &o));
+  xcs_plan pl;
+  XCS_TRY(given_begin(&pl, 0));
+  int nz = 0;
+  if (e == ENS_SMV) {
+    if (!col_ptr || !rows || !vals) return set_err(XCS_EINVAL, "smv arrays required");
+    nz = col_ptr[n];
+  }
+  XCS_TRY(plan_alloc(&pl, ALG_NIHT, e, 0, m, n, 0, nz, &o));
+  XProblem &P = pl.h;
+  if (e == ENS_GEN) {
+    if (!A) return set_err(XCS_EINVAL, "A required");
+    float *dA;
+    XCS_TRY(pl.alloc(&dA, (size_t)P.lda * n));
+    HIP_TRY(hipMemcpy2DAsync(dA, sizeof(float) * P.lda, A, sizeof(float) * m, sizeof(float) * m, n,
+                             hipMemcpyHostToDevice, pl.stream));
+    P.A = dA;
+  } else if (e == ENS_SMV) {
+    std::vector<int> cp(col_ptr, col_ptr + n + 1), rr(rows, rows + nz);
+    std::vector<float> vv(vals, vals + nz);
+    for (int q = 0; q < nz; q++)
+      if (rr[q] < 0 || rr[q] >= m) return set_err(XCS_EINVAL, "smv row out of range");
+    XCS_TRY(plan_set_smv(&pl, cp, rr, vv));
+  } else {
+    if (!dct_rows) return set_err(XCS_EINVAL, "dct rows required");
+    for (int i = 0; i < m; i++)
+      if (dct_rows[i] < 0 || dct_rows[i] >= n) return set_err(XCS_EINVAL, "dct row out of range");
+    int *dr;
+    XCS_TRY(pl.alloc(&dr, (size_t)m));
+    HIP_TRY(hipMemcpyAsync(dr, dct_rows, sizeof(int) * m, hipMemcpyHostToDevice, pl.stream));
+    P.dct_rows = dr;
+  }
+  const int lin = transpose ? m : n, lout = transpose ? n : m;
+  float *din, *dout;
+  XCS_TRY(pl.alloc(&din, (size_t)std::max(lin, P.lda)));
+  XCS_TRY(pl.alloc(&dout, (size_t)std::max(lout, P.lda)));
+  HIP_TRY(hipMemcpyAsync(din, in, sizeof(float) * lin, hipMemcpyHostToDevice, pl.stream));
+  int cnt = 0;
+  if (!transpose) {
+    std::vector<int> supp;
+    for (int j = 0; j < n; j++)
+      if (in[j] != 0.f) supp.push_back(j);
+    cnt = (int)supp.size();
+    if (cnt) HIP_TRY(hipMemcpyAsync(P.supp, supp.data(), sizeof(int) * cnt, hipMemcpyHostToDevice, pl.stream));
+  }
+  XCS_TRY(plan_upload_desc(&pl));
+  XCS_TRY(launch_apply(&pl, transpose, din, dout, cnt));
+  HIP_TRY(hipStreamSynchronize(pl.stream));
+  HIP_TRY(hipMemcpy(out, dout, sizeof(float) * lout, hipMemcpyDeviceToHost));
+  return XCS_OK;
+}
+
+int xcs_generate_gen(int m, int n, unsigned int seed, unsigned int instance, float *A) {
+  if (m <= 0 || n <= 0 || !A) return set_err(XCS_EINVAL, "bad arguments");
+  XCS_TRY(use_device(0));
+  xcs_plan pl;
+  HIP_TRY(hipStreamCreateWithFlags(&pl.stream, hipStreamNonBlocking));
+  float *dA;
+  XCS_TRY(pl.alloc(&dA, (size_t)m * n));
+  const size_t total = (size_t)m * n;
+  const int blocks = (int)std::min<size_t>(65536, (total + 255) / 256);
+  hipLaunchKernelGGL(gen_matrix_kernel, dim3(blocks), dim3(256), 0, pl.stream, dA, m, n, m, seed,
+                     instance);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(pl.stream));
+  HIP_TRY(hipMemcpy(A, dA, sizeof(float) * total, hipMemcpyDeviceToHost));
+  return XCS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,237 @@
+// Device primitives shared by every kernel of the recovery engine (gfx950).
+//
+//  * Team: the set of workgroups that cooperate on one problem instance.
+//    size == 1: one workgroup owns the instance (batched engine, barrier is
+//    __syncthreads).  size > 1: the whole cooperative grid owns it (single
+//    large instance, barrier is the cooperative-groups grid barrier).
+//  * Deterministic reductions: lane butterflies, then LDS, then (size > 1) a
+//    fixed-order pass over per-workgroup partials that every workgroup performs
+//    identically, so every thread of every workgroup holds the same bits.
+//  * Support selection with the reference's exact float semantics
+//    (GAGA/gaga/kernels.cu:135-146, GAGA/gaga/functions.cu:397-436).
+//  * Philox4x32-10 (Salmon et al., SC'11) for on-device problem generation.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <hip/hip_cooperative_groups.h>
+#include <stdint.h>
+
+namespace xcs {
+
+constexpr int WAVE = 64;
+constexpr int MAX_WAVES = 16;  // blockDim.x <= 1024
+
+// ---------------------------------------------------------------- team ----
+struct Team {
+  int rank;  // workgroup rank inside the team
+  int size;  // number of workgroups in the team
+  // global scratch for cross-workgroup partials: two alternating banks of
+  // RED_SLOTS * size doubles (bank b at red + b * RED_SLOTS * size)
+  double *red;
+  int bank;
+  __device__ int gtid() const { return rank * (int)blockDim.x + (int)threadIdx.x; }
+  __device__ int gsize() const { return size * (int)blockDim.x; }
+  __device__ int gwave() const { return rank * ((int)blockDim.x / WAVE) + (int)threadIdx.x / WAVE; }
+  __device__ int nwaves() const { return size * ((int)blockDim.x / WAVE); }
+  __device__ void sync() const {
+    if (size > 1)
+      cooperative_groups::this_grid().sync();
+    else
+      __syncthreads();
+  }
+};
+constexpr int RED_SLOTS = 8;
+
+// ---------------------------------------------------------- reductions ----
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int o = WAVE / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+template <typename T>
+__device__ __forceinline__ T wave_max(T v) {
+#pragma unroll
+  for (int o = WAVE / 2; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+  return v;
+}
+__device__ __forceinline__ int wave_sum_i(int v) {
+#pragma unroll
+  for (int o = WAVE / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+// Reduce K values across the team: op[k] == 0 sum, 1 max.  On return every
+// thread of every workgroup of the team holds the same results in v[].
+// Contains a barrier when size > 1 (and always a __syncthreads).
+template <typename T, int K>
+__device__ void team_reduce(Team &t, T (&v)[K], const int (&op)[K]) {
+  __shared__ T sm[K][MAX_WAVES];
+  const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
+  const int nw = blockDim.x / WAVE;
+#pragma unroll
+  for (int k = 0; k < K; k++) v[k] = op[k] ? wave_max(v[k]) : wave_sum(v[k]);
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < K; k++) sm[k][wave] = v[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    T a = sm[k][0];
+    for (int w = 1; w < nw; w++) a = op[k] ? (T)fmax(a, sm[k][w]) : a + sm[k][w];
+    v[k] = a;
+  }
+  __syncthreads();
+  if (t.size > 1) {
+    double *buf = t.red + (size_t)t.bank * RED_SLOTS * t.size;
+    t.bank ^= 1;
+    if (threadIdx.x == 0) {
+#pragma unroll
+      for (int k = 0; k < K; k++) buf[k * t.size + t.rank] = (double)v[k];
+    }
+    t.sync();
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      T a = op[k] ? (T)0 : (T)0;
+      for (int i = threadIdx.x; i < t.size; i += blockDim.x) {
+        T b = (T)buf[k * t.size + i];
+        a = op[k] ? (T)fmax(a, b) : a + b;
+      }
+      v[k] = a;
+    }
+    // second level: same fixed shape as the first
+#pragma unroll
+    for (int k = 0; k < K; k++) v[k] = op[k] ? wave_max(v[k]) : wave_sum(v[k]);
+    if (lane == 0) {
+#pragma unroll
+      for (int k = 0; k < K; k++) sm[k][wave] = v[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      T a = sm[k][0];
+      for (int w = 1; w < nw; w++) a = op[k] ? (T)fmax(a, sm[k][w]) : a + sm[k][w];
+      v[k] = a;
+    }
+    __syncthreads();
+  }
+}
+
+// Workgroup-wide exclusive scan of one int per thread; returns the exclusive
+// prefix, *total receives the workgroup sum.
+__device__ __forceinline__ int wg_exclusive_scan(int v, int *total) {
+  __shared__ int ws[MAX_WAVES + 1];
+  const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
+  const int nw = blockDim.x / WAVE;
+  int incl = v;
+#pragma unroll
+  for (int o = 1; o < WAVE; o <<= 1) {
+    int u = __shfl_up(incl, o);
+    if (lane >= o) incl += u;
+  }
+  if (lane == WAVE - 1) ws[wave] = incl;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int s = 0;
+    for (int w = 0; w < nw; w++) {
+      int c = ws[w];
+      ws[w] = s;
+      s += c;
+    }
+    ws[MAX_WAVES] = s;
+  }
+  __syncthreads();
+  int r = ws[wave] + incl - v;
+  *total = ws[MAX_WAVES];
+  __syncthreads();
+  return r;
+}
+
+// ------------------------------------------------------ support selection --
+// LinearBinning's per-entry bin (GAGA/gaga/kernels.cu:135-146), exact float
+// semantics: the `intercept*.000001` literal is a double so that comparison is
+// done in double; slope*(intercept-temp) is a rounded subtract then a rounded
+// multiply; the small-entry branch is the fused multiply-add nvcc contracts.
+__device__ __forceinline__ int linear_bin(float v, float slope, float intercept) {
+  float temp = fabsf(v);
+  if ((double)temp > (double)intercept * .000001) {
+    float d = __fsub_rn(intercept, temp);
+    float b = __fmul_rn(slope, d);
+    b = fmaxf(0.0f, b);
+    return (int)b;
+  }
+  return (int)__fmaf_rn(slope, intercept, 1.0f);
+}
+
+// The sequential k-bin walk of FindSupportSet (functions.cu:414-420) done as a
+// workgroup-parallel scan over counters[0, max_bin).  Every workgroup that
+// calls it computes the same (k_bin, sum).  counters must be complete.
+__device__ void kbin_select(const int *counters, int max_bin, int k, int *k_bin_out,
+                            int *sum_out) {
+  __shared__ int res[2];
+  if (k <= 0) {  // the reference loop body never runs: k_bin = -1, sum = 0
+    *k_bin_out = -1;
+    *sum_out = 0;
+    return;
+  }
+  const int nt = blockDim.x;
+  const int chunk = (max_bin + nt - 1) / nt;
+  const int lo = threadIdx.x * chunk;
+  const int hi = min(lo + chunk, max_bin);
+  int s = 0;
+  for (int b = lo; b < hi; b++) s += counters[b];
+  int total;
+  int before = wg_exclusive_scan(s, &total);
+  if (threadIdx.x == 0) {
+    res[0] = max_bin - 1;  // never reaches k: k_bin = max_bin - 1, sum = total
+    res[1] = total;
+  }
+  __syncthreads();
+  if (before < k && before + s >= k) {  // exactly one thread owns the crossing
+    int acc = before;
+    int b = lo;
+    for (; b < hi; b++) {
+      acc += counters[b];
+      if (acc >= k) break;
+    }
+    res[0] = b;
+    res[1] = acc;
+  }
+  __syncthreads();
+  *k_bin_out = res[0];
+  *sum_out = res[1];
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------- philox ---
+__host__ __device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2,
+                                                       uint32_t c3, uint32_t k0, uint32_t k1,
+                                                       uint32_t out[4]) {
+#pragma unroll
+  for (int r = 0; r < 10; r++) {
+    uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+    uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+    uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+    uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+    c1 = (uint32_t)p1;
+    c3 = (uint32_t)p0;
+    c0 = n0;
+    c2 = n2;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  out[0] = c0;
+  out[1] = c1;
+  out[2] = c2;
+  out[3] = c3;
+}
+__host__ __device__ __forceinline__ double u01d(uint32_t v) {
+  return ((double)v + 0.5) * 2.3283064365386963e-10;
+}
+__host__ __device__ __forceinline__ uint32_t bounded_u32(uint32_t v, uint32_t range) {
+  return (uint32_t)(((uint64_t)v * (uint64_t)range) >> 32);
+}
+enum { STREAM_A = 0, STREAM_SUPPORT = 1, STREAM_VALUES = 2, STREAM_SMV_ROWS = 3,
+       STREAM_SMV_VALS = 4, STREAM_DCT_ROWS = 5 };
+
+}  // namespace xcs

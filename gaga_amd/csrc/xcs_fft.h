@@ -1,0 +1,174 @@
+// Team-level power-of-two complex FFT and the subsampled orthonormal DCT
+// operator built on it (the 'dct' ensemble, GAGA/Matrices/dct_gpu.cu:121-353:
+// Makhoul's DCT-II via one complex FFT, rows selected / zero-filled).
+//
+// FFT: four-step decomposition n = n1 * n2 (n1, n2 <= 2048) so every
+// sub-transform runs as an iterative radix-2 transform in LDS:
+//   pass 1: n2 column transforms of length n1 (blocks of adjacent columns so
+//           global accesses are contiguous), times the twiddle w_n^(c*k1);
+//   pass 2: n1 row transforms of length n2, written transposed.
+// Transforms of n <= 4096 run in one workgroup in a single pass.
+// Twiddles come from tables computed in double precision at set-up.
+#pragma once
+#include "xcs_device.h"
+#include "xcs_engine.h"
+
+namespace xcs {
+
+constexpr int RLDS = 8192;   // gen: residual staged in LDS when lda <= RLDS
+constexpr int HLDS = 4096;   // binning: histogram in LDS when max_bin <= HLDS
+constexpr int FLDS = 4096;   // FFT: complex elements held in LDS
+
+union Lds {
+  struct {
+    float4 r[RLDS / 4];
+    int hist[HLDS];
+  } s;
+  float2 f[FLDS];
+};
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(__fsub_rn(__fmul_rn(a.x, b.x), __fmul_rn(a.y, b.y)),
+                     __fadd_rn(__fmul_rn(a.x, b.y), __fmul_rn(a.y, b.x)));
+}
+__device__ __forceinline__ float2 twid(const float2 *tw, int n, long long e, int sign) {
+  // tw[t] = exp(-2 pi i t / n); sign +1 (inverse) conjugates.
+  const int t = (int)(e & (long long)(n - 1));
+  float2 w = tw[t];
+  if (sign > 0) w.y = -w.y;
+  return w;
+}
+
+// cnt sequences of length len (power of two) stored contiguously in LDS at
+// f[s*len + bitrev(j)]; in-place iterative radix-2 DIT.  len*cnt <= FLDS.
+__device__ void lds_fft(float2 *f, int len, int cnt, const float2 *tw, int n, int sign) {
+  const int half = len >> 1;
+  for (int span = 1; span < len; span <<= 1) {
+    const int stride = n / (span << 1);  // w_{2 span}^pos = w_n^(pos * stride)
+    for (int b = threadIdx.x; b < half * cnt; b += blockDim.x) {
+      const int s = b / half, bb = b - s * half;
+      const int grp = bb / span, pos = bb - grp * span;
+      const int i0 = s * len + grp * (span << 1) + pos;
+      const int i1 = i0 + span;
+      const float2 w = twid(tw, n, (long long)pos * stride, sign);
+      const float2 u = f[i0];
+      const float2 v = cmul(f[i1], w);
+      f[i0] = make_float2(__fadd_rn(u.x, v.x), __fadd_rn(u.y, v.y));
+      f[i1] = make_float2(__fsub_rn(u.x, v.x), __fsub_rn(u.y, v.y));
+    }
+    __syncthreads();
+  }
+}
+
+__device__ __forceinline__ int bitrev(int v, int bits) {
+  return (int)(__brev((unsigned)v) >> (32 - bits));
+}
+
+// out = FFT(in) of length n = 2^log2n (sign -1 forward, +1 unnormalised
+// inverse).  in and out distinct, both length n.  Caller makes `in` visible;
+// ends with a team barrier (out visible).  `mid` is scratch of length n.
+__device__ void team_fft(Team &t, Lds &L, const float2 *in, float2 *mid, float2 *out,
+                         const float2 *tw, int n, int log2n, int sign) {
+  if (n <= FLDS) {
+    if (t.rank == 0) {
+      for (int j = threadIdx.x; j < n; j += blockDim.x) L.f[bitrev(j, log2n)] = in[j];
+      __syncthreads();
+      lds_fft(L.f, n, 1, tw, n, sign);
+      for (int j = threadIdx.x; j < n; j += blockDim.x) out[j] = L.f[j];
+      __syncthreads();
+    }
+    t.sync();
+    return;
+  }
+  const int b1 = log2n / 2, b2 = log2n - b1;
+  const int n1 = 1 << b1, n2 = 1 << b2;
+  // pass 1: columns c (stride n2), blocks of cb adjacent columns
+  {
+    const int cb = FLDS / n1;
+    const int nblk = n2 / cb;
+    for (int blk = t.rank; blk < nblk; blk += t.size) {
+      const int c0 = blk * cb;
+      for (int idx = threadIdx.x; idx < cb * n1; idx += blockDim.x) {
+        const int c = idx % cb, j1 = idx / cb;
+        L.f[c * n1 + bitrev(j1, b1)] = in[c0 + c + (size_t)n2 * j1];
+      }
+      __syncthreads();
+      lds_fft(L.f, n1, cb, tw, n, sign);
+      for (int idx = threadIdx.x; idx < cb * n1; idx += blockDim.x) {
+        const int c = idx % cb, k1 = idx / cb;
+        const float2 w = twid(tw, n, (long long)(c0 + c) * k1, sign);
+        mid[c0 + c + (size_t)n2 * k1] = cmul(L.f[c * n1 + k1], w);
+      }
+      __syncthreads();
+    }
+  }
+  t.sync();
+  // pass 2: rows k1 (contiguous, length n2), written transposed
+  {
+    const int rb = FLDS / n2;
+    const int nblk = n1 / rb;
+    for (int blk = t.rank; blk < nblk; blk += t.size) {
+      const int r0 = blk * rb;
+      for (int idx = threadIdx.x; idx < rb * n2; idx += blockDim.x) {
+        const int r = idx / n2, j2 = idx - r * n2;
+        L.f[r * n2 + bitrev(j2, b2)] = mid[(size_t)(r0 + r) * n2 + j2];
+      }
+      __syncthreads();
+      lds_fft(L.f, n2, rb, tw, n, sign);
+      for (int idx = threadIdx.x; idx < rb * n2; idx += blockDim.x) {
+        const int r = idx % rb, k2 = idx / rb;
+        out[(r0 + r) + (size_t)n1 * k2] = L.f[r * n2 + k2];
+      }
+      __syncthreads();
+    }
+  }
+  t.sync();
+}
+
+// DCT weights: w_0 = sqrt(1/n), w_q = sqrt(2/n) (dct_gpu.cu:170-176).
+__device__ __forceinline__ float dct_w(const XProblem &P, int q) {
+  return q == 0 ? (float)sqrt(1.0 / (double)P.n) : (float)sqrt(2.0 / (double)P.n);
+}
+
+// y = (DCT-II_ortho v)[rows]  (A_dct, dct_gpu.cu:328-338).  Strided out.
+__device__ void dct_A(Team &t, Lds &L, const XProblem &P, const float *v, float *out) {
+  const int n = P.n;
+  float2 *z = P.cwork, *mid = P.cwork + n, *Z = P.cwork + 2 * (size_t)n;
+  const float2 *qw = P.twiddle + n;  // (cos, sin)(pi q / 2n)
+  for (int q = t.gtid(); q < n / 2; q += t.gsize()) {
+    z[q] = make_float2(v[2 * q], 0.f);
+    z[n - 1 - q] = make_float2(v[2 * q + 1], 0.f);
+  }
+  t.sync();
+  team_fft(t, L, z, mid, Z, P.twiddle, n, P.log2n, -1);
+  for (int i = t.gtid(); i < P.m; i += t.gsize()) {
+    const int q = P.dct_rows[i];
+    const float2 c = Z[q], f = qw[q];
+    out[i] = __fmul_rn(dct_w(P, q), __fadd_rn(__fmul_rn(c.x, f.x), __fmul_rn(c.y, f.y)));
+  }
+}
+
+// g = DCT-III_ortho(zero-fill(r at rows))  (AT_dct, dct_gpu.cu:341-353).
+// epi(j, g_j) by the owner of j (strided over pairs).
+template <class Epi>
+__device__ void dct_AT(Team &t, Lds &L, const XProblem &P, const float *r, Epi &&epi) {
+  const int n = P.n;
+  float2 *z = P.cwork, *mid = P.cwork + n, *Y = P.cwork + 2 * (size_t)n;
+  const float2 *qw = P.twiddle + n;
+  for (int q = t.gtid(); q < n; q += t.gsize()) z[q] = make_float2(0.f, 0.f);
+  t.sync();
+  for (int i = t.gtid(); i < P.m; i += t.gsize()) {
+    const int q = P.dct_rows[i];
+    const float d = __fmul_rn(dct_w(P, q), r[i]);
+    const float2 f = qw[q];
+    z[q] = make_float2(__fmul_rn(d, f.x), __fmul_rn(d, f.y));
+  }
+  t.sync();
+  team_fft(t, L, z, mid, Y, P.twiddle, n, P.log2n, +1);
+  for (int q = t.gtid(); q < n / 2; q += t.gsize()) {
+    epi(2 * q, Y[q].x);
+    epi(2 * q + 1, Y[n - 1 - q].x);
+  }
+}
+
+}  // namespace xcs

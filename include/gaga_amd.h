@@ -1,0 +1,121 @@
+/*
+ * gaga_amd: MI355X-native greedy sparse recovery (NIHT, HTP, CSMPSP, CGIHT over
+ * the 'gen', 'smv' and 'dct' ensembles).  C ABI; plain pointers and sizes.
+ *
+ * The entry points mirror the reference's MATLAB/MEX call surface
+ *   gaga_cs(alg, ens, k, m, n[, p], options)          GAGA/gaga/gaga_cs.m:7-64
+ * whose drivers are the MEX functions in GAGA/gaga/gaga_gen.cu:41,
+ * GAGA/gaga/gaga_smv.cu:38 and GAGA/gaga/gaga_dct.cu.  The reference's licence
+ * (GAGA/GAGA_license:5) reserves the names gaga_cs / gaga_gen / gaga_smv /
+ * gaga_dct, so this library exports the same surface under the xcs_ prefix.
+ *
+ * Conventions (all entry points):
+ *   - return 0 on success, a negative XCS_E* code on error; xcs_last_error()
+ *     gives a message (thread-local).  Nothing calls exit().
+ *   - host buffers are caller-owned; device memory is owned by the library
+ *     unless an entry point says it takes device pointers.
+ *   - index arrays of the user-problem forms are 1-based, as MATLAB passes
+ *     them (the reference shifts them with indexShiftDown, kernels.cu:155-159).
+ *   - calls are synchronous and thread-safe (one HIP stream per call).
+ */
+#ifndef GAGA_AMD_H
+#define GAGA_AMD_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  XCS_OK = 0,
+  XCS_EINVAL = -1,   /* bad argument (algorithm, ensemble, size, option) */
+  XCS_EHIP = -2,     /* HIP runtime error */
+  XCS_ENOMEM = -3,   /* device allocation failed */
+  XCS_ENODEV = -4,   /* no usable GPU */
+  XCS_EUNSUP = -5    /* valid request this build does not implement */
+};
+
+/* Options: the reference's gagaOptions list (GAGA/gaga/gagaOptions.m:8-58,
+ * parsed at GAGA/gaga/gaga_gen.cu:205-320) plus this build's execution knobs.
+ * Always initialise with xcs_options_default(). */
+typedef struct xcs_options {
+  float tol;                 /* stop when ||y - A x||_2 <= tol (reference 'tol') */
+  int tol_reference_default; /* 1: use the reference GPU default tol = 10^(-4) in C,
+                                which evaluates to -10 (gaga_gen.cu:183); never stops */
+  int maxiter;               /* 'maxiter'; default 300 for HTP/CSMPSP, 5000 otherwise */
+  int num_bins;              /* 'numBins'; default max(n/20, 1000) */
+  int restart;               /* 'restartFlag' for CGIHT (0 off, 1 on) */
+  unsigned int seed;         /* 'seed' (generator key word 0) */
+  unsigned int instance;     /* generator key word 1 (instance id) */
+  int vec_distribution;      /* 'vecDistribution': 0 uniform, 1 binary (default), 2 gaussian */
+  int matrix_ensemble;       /* 'matrixEnsemble': gen 1 gaussian; smv 1 ones/sqrt(p),
+                                2 binary +-1/sqrt(p), 3 ones (reference smv default) */
+  int gpu_number;            /* 'gpuNumber' */
+  int mode;                  /* 0 device-resident solve (one launch); 1 plain form:
+                                one outer iteration per launch, host-side stop test */
+  int team_size;             /* workgroups cooperating on one instance; 0 = auto */
+  int block_size;            /* threads per workgroup; 0 = auto */
+} xcs_options;
+
+/* Fill *opts with the reference defaults for (alg, ens, n). */
+int xcs_options_default(const char *alg, const char *ens, int n, xcs_options *opts);
+
+/* Generate-and-solve form: [norms times iterations support convRate xout] =
+ * gaga_cs(alg, ens, k, m, n[, p], options)  (gaga_gen.cu:90-114, gaga_smv.cu:131-158).
+ *   norms[3]   relative l1, l2, linf error of x_out vs the planted x (results.cu:444-487)
+ *   times[3]   solve ms, ms per iteration, total ms including generation
+ *   support[4] TP, FP, TN, FN (results.cu:400-430)
+ *   x_out[n]   recovered vector (may be NULL); x_true[n] planted vector (may be NULL)
+ * p is the smv nonzeros per column (ignored for gen/dct). */
+int xcs_cs_generate(const char *alg, const char *ens, int k, int m, int n, int p,
+                    const xcs_options *opts, float norms[3], float times[3], int *iterations,
+                    int support[4], float *conv_rate, float *x_out, float *x_true);
+
+/* Solve-given-problem forms ([outputVector iterations convRate] = gaga_cs(alg,
+ * 'gen', y, A, k, options), gaga_gen.cu:115-134).  A is column-major m x n. */
+int xcs_cs_gen(const char *alg, const float *y, const float *A, int m, int n, int k,
+               const xcs_options *opts, float *x_out, int *iterations, float *conv_rate);
+/* smv: (rows, cols, vals) triplets, nz entries, 1-based (gaga_smv.cu:159-195). */
+int xcs_cs_smv(const char *alg, const float *y, const int *rows, const int *cols,
+               const float *vals, int nz, int m, int n, int k, const xcs_options *opts,
+               float *x_out, int *iterations, float *conv_rate);
+/* dct: m sampled rows of the orthonormal n-point DCT-II, 1-based (gaga_dct.cu:112-117);
+ * n must be a power of two. */
+int xcs_cs_dct(const char *alg, const float *y, const int *rows, int k, int m, int n,
+               const xcs_options *opts, float *x_out, int *iterations, float *conv_rate);
+
+/* ---- resident plans: inputs generated into / kept in HBM, solves timed on device ---- */
+typedef struct xcs_plan xcs_plan;
+/* Generate one instance on the device (seed/instance from opts) and keep it. */
+int xcs_plan_create(const char *alg, const char *ens, int k, int m, int n, int p,
+                    const xcs_options *opts, xcs_plan **plan);
+/* Solve from x = 0 with the inputs already resident; *ms = device time. */
+int xcs_plan_solve(xcs_plan *plan, float *ms);
+/* Results of the last solve (any pointer may be NULL). */
+int xcs_plan_results(xcs_plan *plan, float norms[3], int *iterations, int support[4],
+                     float *conv_rate, int *stop_reason, float *x_out, float *x_true);
+/* Per-iteration trace of the last solve: k_bin and residual norm, length iterations+1. */
+int xcs_plan_trace(xcs_plan *plan, int *k_bin, float *resid_norm, int len);
+void xcs_plan_destroy(xcs_plan *plan);
+
+/* ---- kernel-level entry points (host buffers; used by parity tests) ---- */
+/* One FindSupportSet call (functions.cu:397-436) on the device:
+ * in/out state: *min_val, *alpha, *max_bin, *k_bin; out: bins[n], *sum
+ * (*sum = -1 when the skip branch is taken). */
+int xcs_find_support_set(const float *v, int n, int k, int num_bins, float slope,
+                         float intercept, float max_change, float *min_val, float *alpha,
+                         int *max_bin, int *k_bin, int *bins, int *sum);
+/* y = A x (transpose = 0, x length n) or g = A^T r (transpose = 1, r length m) for one
+ * ensemble; gen: A column-major m x n; smv: 0-based CSC (col_ptr[n+1], rows, vals);
+ * dct: 0-based rows[m]. Unused pointers may be NULL. */
+int xcs_apply(const char *ens, int transpose, int m, int n, const float *A, const int *col_ptr,
+              const int *rows, const float *vals, const int *dct_rows, const float *in,
+              float *out);
+/* Device generator (Philox4x32-10): gen A column-major m x n with N(0, 1/m) entries. */
+int xcs_generate_gen(int m, int n, unsigned int seed, unsigned int instance, float *A);
+
+const char *xcs_last_error(void);
+const char *xcs_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,372 @@
+/*
+ * CPU oracle (test infrastructure only; see oracle.h for scope and pins).
+ * Build: oracle/Makefile (gcc -O2 -ffp-contract=off -fopenmp).
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Philox4x32-10, from Salmon, Moraes, Dror, Shaw, "Parallel random numbers: */
+/* as easy as 1, 2, 3" (SC'11): round multipliers 0xD2511F53 / 0xCD9E8D57,    */
+/* Weyl key increments 0x9E3779B9 / 0xBB67AE85, ten rounds.                   */
+/* ------------------------------------------------------------------------ */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+  uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+  uint32_t k0 = key[0], k1 = key[1];
+  for (int r = 0; r < 10; r++) {
+    uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+    uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+    uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+    uint32_t n0 = hi1 ^ c1 ^ k0;
+    uint32_t n2 = hi0 ^ c3 ^ k1;
+    c0 = n0;
+    c1 = lo1;
+    c2 = n2;
+    c3 = lo0;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  out[0] = c0;
+  out[1] = c1;
+  out[2] = c2;
+  out[3] = c3;
+}
+
+static void draw(uint32_t seed, uint32_t inst, uint32_t a, uint32_t b, uint32_t stream,
+                 uint32_t out[4]) {
+  uint32_t ctr[4] = {a, b, stream, 0u};
+  uint32_t key[2] = {seed, inst};
+  orc_philox4x32_10(ctr, key, out);
+}
+
+/* (0,1) doubles from 32-bit words; Box-Muller (cosine branch). */
+static double u01d(uint32_t v) { return ((double)v + 0.5) * 2.3283064365386963e-10; }
+static double box_muller(uint32_t a, uint32_t b) {
+  return sqrt(-2.0 * log(u01d(a))) * cos(6.283185307179586476925286766559 * u01d(b));
+}
+/* unbiased-enough integer in [0, range) by 32x32->64 multiply (Lemire). */
+static uint32_t bounded(uint32_t v, uint32_t range) {
+  return (uint32_t)(((uint64_t)v * (uint64_t)range) >> 32);
+}
+
+void orc_gen_matrix(int m, int n, int lda, uint32_t seed, uint32_t inst, float *A) {
+  double s = 1.0 / sqrt((double)m);
+#pragma omp parallel for schedule(static)
+  for (int j = 0; j < n; j++) {
+    for (int i = 0; i < lda; i++) {
+      float v = 0.0f;
+      if (i < m) {
+        uint64_t e = (uint64_t)j * (uint64_t)m + (uint64_t)i;
+        uint32_t o[4];
+        draw(seed, inst, (uint32_t)e, (uint32_t)(e >> 32), ORC_STREAM_A, o);
+        v = (float)(box_muller(o[0], o[1]) * s);
+      }
+      A[(size_t)j * (size_t)lda + (size_t)i] = v;
+    }
+  }
+}
+
+/* Floyd's sampling of a k-subset of [0,n), stream `stream`, ascending. */
+static void floyd_subset(int n, int k, uint32_t seed, uint32_t inst, uint32_t stream, int *idx) {
+  unsigned char *sel = (unsigned char *)calloc((size_t)n, 1);
+  for (int d = 0; d < k; d++) {
+    int j = n - k + d;
+    uint32_t o[4];
+    draw(seed, inst, (uint32_t)d, 0u, stream, o);
+    int t = (int)bounded(o[0], (uint32_t)(j + 1));
+    if (sel[t])
+      sel[j] = 1;
+    else
+      sel[t] = 1;
+  }
+  int c = 0;
+  for (int i = 0; i < n; i++)
+    if (sel[i]) idx[c++] = i;
+  free(sel);
+}
+
+void orc_support(int n, int k, uint32_t seed, uint32_t inst, int *idx) {
+  floyd_subset(n, k, seed, inst, ORC_STREAM_SUPPORT, idx);
+}
+
+void orc_sparse_vector(int n, int k, int dist, uint32_t seed, uint32_t inst, float *x) {
+  int *idx = (int *)malloc(sizeof(int) * (size_t)(k > 0 ? k : 1));
+  memset(x, 0, sizeof(float) * (size_t)n);
+  orc_support(n, k, seed, inst, idx);
+  for (int t = 0; t < k; t++) {
+    int i = idx[t];
+    uint32_t o[4];
+    draw(seed, inst, (uint32_t)i, 0u, ORC_STREAM_VALUES, o);
+    float v;
+    if (dist == 1)
+      v = (o[0] & 1u) ? 1.0f : -1.0f;
+    else if (dist == 2)
+      v = (float)box_muller(o[0], o[1]);
+    else
+      v = (float)(((o[0] >> 8) + 0.5) * (1.0 / 16777216.0));
+    x[i] = v;
+  }
+  free(idx);
+}
+
+int orc_smv_matrix(int m, int n, int p, int ens, uint32_t seed, uint32_t inst, int *rows,
+                   float *vals) {
+  float scale = 1.0f / sqrtf((float)p);
+  int *cnt = (int *)calloc((size_t)m, sizeof(int));
+  for (int j = 0; j < n; j++) {
+    int *r = rows + (size_t)j * p;
+    int got = 0;
+    for (uint32_t t = 0; got < p; t++) {
+      uint32_t o[4];
+      draw(seed, inst, (uint32_t)j, t, ORC_STREAM_SMV_ROWS, o);
+      int cand = (int)bounded(o[0], (uint32_t)m);
+      int dup = 0;
+      for (int q = 0; q < got; q++)
+        if (r[q] == cand) dup = 1;
+      if (!dup) r[got++] = cand;
+    }
+    /* ascending rows within a column (insertion sort, p is small) */
+    for (int a = 1; a < p; a++) {
+      int v = r[a], b = a - 1;
+      while (b >= 0 && r[b] > v) {
+        r[b + 1] = r[b];
+        b--;
+      }
+      r[b + 1] = v;
+    }
+    for (int t = 0; t < p; t++) {
+      float v = 1.0f;
+      if (ens == 2) {
+        uint32_t o[4];
+        draw(seed, inst, (uint32_t)j, (uint32_t)t, ORC_STREAM_SMV_VALS, o);
+        v = (o[0] & 1u) ? 1.0f : -1.0f;
+      }
+      if (ens == 1 || ens == 2) v = v * scale;
+      vals[(size_t)j * p + t] = v;
+      cnt[r[t]]++;
+    }
+  }
+  int empty = 0;
+  for (int i = 0; i < m; i++)
+    if (cnt[i] == 0) empty++;
+  free(cnt);
+  return empty;
+}
+
+void orc_dct_rows(int m, int n, uint32_t seed, uint32_t inst, int *rows) {
+  floyd_subset(n, m, seed, inst, ORC_STREAM_DCT_ROWS, rows);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Operators (double).                                                       */
+/* ------------------------------------------------------------------------ */
+/* complex radix-2 FFT in place, sign -1 forward, +1 inverse (unnormalised) */
+static void fft_inplace(int n, double *re, double *im, int sign) {
+  for (int i = 1, j = 0; i < n; i++) {
+    int bit = n >> 1;
+    for (; j & bit; bit >>= 1) j ^= bit;
+    j ^= bit;
+    if (i < j) {
+      double t = re[i]; re[i] = re[j]; re[j] = t;
+      t = im[i]; im[i] = im[j]; im[j] = t;
+    }
+  }
+  for (int len = 2; len <= n; len <<= 1) {
+    int half = len >> 1;
+    double ang = sign * 2.0 * M_PI / len;
+    for (int i = 0; i < n; i += len) {
+      for (int q = 0; q < half; q++) {
+        double wr = cos(ang * q), wi = sin(ang * q);
+        double ar = re[i + q + half], ai = im[i + q + half];
+        double tr = ar * wr - ai * wi, ti = ar * wi + ai * wr;
+        re[i + q + half] = re[i + q] - tr;
+        im[i + q + half] = im[i + q] - ti;
+        re[i + q] += tr;
+        im[i + q] += ti;
+      }
+    }
+  }
+}
+
+/* Makhoul's DCT-II via one complex FFT (dct_gpu.cu:121-182 structure). */
+void orc_dct2_ortho(int n, const double *x, double *X) {
+  double *re = (double *)malloc(sizeof(double) * (size_t)n);
+  double *im = (double *)calloc((size_t)n, sizeof(double));
+  for (int q = 0; q < n / 2; q++) {
+    re[q] = x[2 * q];
+    re[n - 1 - q] = x[2 * q + 1];
+  }
+  fft_inplace(n, re, im, -1);
+  for (int q = 0; q < n; q++) {
+    double f = M_PI * q / (2.0 * n);
+    double w = (q == 0) ? sqrt(1.0 / n) : sqrt(2.0 / n);
+    X[q] = w * (re[q] * cos(f) + im[q] * sin(f));
+  }
+  free(re);
+  free(im);
+}
+
+/* Orthonormal DCT-III (dct_gpu.cu:142-153, 186-206 structure). */
+void orc_dct3_ortho(int n, const double *X, double *x) {
+  double *re = (double *)malloc(sizeof(double) * (size_t)n);
+  double *im = (double *)malloc(sizeof(double) * (size_t)n);
+  for (int q = 0; q < n; q++) {
+    double f = M_PI * q / (2.0 * n);
+    double w = (q == 0) ? sqrt(1.0 / n) : sqrt(2.0 / n);
+    re[q] = w * X[q] * cos(f);
+    im[q] = w * X[q] * sin(f);
+  }
+  fft_inplace(n, re, im, +1);
+  for (int q = 0; q < n / 2; q++) {
+    x[2 * q] = re[q];
+    x[2 * q + 1] = re[n - 1 - q];
+  }
+  free(re);
+  free(im);
+}
+
+void orc_apply_A64(const orc_matrix *M, const double *x, double *y) {
+  int m = M->m, n = M->n;
+  if (M->ens == ORC_ENS_GEN) {
+    /* gen_matvec_sparse (gen_gpu.cu:79-132): only nonzero x_j contribute */
+#pragma omp parallel
+    {
+#ifdef _OPENMP
+      int nt = omp_get_num_threads(), t = omp_get_thread_num();
+#else
+      int nt = 1, t = 0;
+#endif
+      int lo = (int)((long long)m * t / nt), hi = (int)((long long)m * (t + 1) / nt);
+      for (int i = lo; i < hi; i++) y[i] = 0.0;
+      for (int j = 0; j < n; j++) {
+        double xj = x[j];
+        if (xj == 0.0) continue;
+        const float *col = M->A + (size_t)j * (size_t)M->lda;
+        for (int i = lo; i < hi; i++) y[i] += (double)col[i] * xj;
+      }
+    }
+  } else if (M->ens == ORC_ENS_SMV) {
+    /* sparse_matvec (smv_gpu.cu:29-37) */
+    for (int i = 0; i < m; i++) y[i] = 0.0;
+    for (int j = 0; j < n; j++) {
+      double xj = x[j];
+      if (xj == 0.0) continue;
+      for (int e = M->col_ptr[j]; e < M->col_ptr[j + 1]; e++) y[M->rows[e]] += (double)M->vals[e] * xj;
+    }
+  } else {
+    /* A_dct (dct_gpu.cu:328-338): rows of the orthonormal DCT-II */
+    double *X = (double *)malloc(sizeof(double) * (size_t)n);
+    orc_dct2_ortho(n, x, X);
+    for (int i = 0; i < m; i++) y[i] = X[M->dct_rows[i]];
+    free(X);
+  }
+}
+
+void orc_apply_AT64(const orc_matrix *M, const double *r, double *g) {
+  int m = M->m, n = M->n;
+  if (M->ens == ORC_ENS_GEN) {
+    /* gen_matvec on the transpose (gen_gpu.cu:33-74, 161-171) */
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < n; j++) {
+      const float *col = M->A + (size_t)j * (size_t)M->lda;
+      double s = 0.0;
+      for (int i = 0; i < m; i++) s += (double)col[i] * r[i];
+      g[j] = s;
+    }
+  } else if (M->ens == ORC_ENS_SMV) {
+    /* sparse_matvecT (smv_gpu.cu:40-47) */
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < n; j++) {
+      double s = 0.0;
+      for (int e = M->col_ptr[j]; e < M->col_ptr[j + 1]; e++) s += (double)M->vals[e] * r[M->rows[e]];
+      g[j] = s;
+    }
+  } else {
+    /* AT_dct (dct_gpu.cu:341-353): zero fill, scatter rows, DCT-III */
+    double *z = (double *)calloc((size_t)n, sizeof(double));
+    for (int i = 0; i < m; i++) z[M->dct_rows[i]] = r[i];
+    orc_dct3_ortho(n, z, g);
+    free(z);
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Results (results.cu:363-530).                                             */
+/* ------------------------------------------------------------------------ */
+void orc_results(int n, int k, const float *x_true, const float *x_out, double norms[3],
+                 int support[4]) {
+  double a1 = 0, a2 = 0, ai = 0, d1 = 0, d2 = 0, di = 0;
+  int tp = 0, tn = 0;
+  for (int i = 0; i < n; i++) {
+    double t = x_true[i], o = x_out[i], d = t - o;
+    a1 += fabs(t);
+    a2 += t * t;
+    if (fabs(t) > ai) ai = fabs(t);
+    d1 += fabs(d);
+    d2 += d * d;
+    if (fabs(d) > di) di = fabs(d);
+    if (t != 0.0) {
+      if (o != 0.0) tp++;
+    } else if (o == 0.0) {
+      tn++;
+    }
+  }
+  norms[0] = d1 / a1;
+  norms[1] = sqrt(d2) / sqrt(a2);
+  norms[2] = di / ai;
+  support[0] = tp;
+  support[1] = k - tp;
+  support[2] = tn;
+  support[3] = n - k - tn;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Precision-generic solvers.                                                */
+/* ------------------------------------------------------------------------ */
+#define REAL float
+#define REAL_IS_DOUBLE 0
+#define PFX orc32_
+#define FMA fmaf
+#define FABS fabsf
+#define FMAX fmaxf
+#define POW powf
+#define SQRT sqrtf
+#include "solver_impl.h"
+#undef REAL
+#undef REAL_IS_DOUBLE
+#undef PFX
+#undef FMA
+#undef FABS
+#undef FMAX
+#undef POW
+#undef SQRT
+
+#define REAL double
+#define REAL_IS_DOUBLE 1
+#define PFX orc64_
+#define FMA fma
+#define FABS fabs
+#define FMAX fmax
+#define POW pow
+#define SQRT sqrt
+#include "solver_impl.h"
+
+/* float-exact public entry points for the selection primitives */
+void orc_linear_binning(const float *v, int n, int num_bins, int max_bin, float slope,
+                        float intercept, int *bins, int *counters) {
+  orc32_linear_binning(v, n, num_bins, max_bin, slope, intercept, bins, counters);
+}
+
+int orc_find_support_set(const float *v, int *bins, int *counters, float slope,
+                         float intercept, float max_change, float *min_val, float *alpha,
+                         int *max_bin, int *k_bin, int n, int k, int num_bins) {
+  return orc32_find_support_set(v, bins, counters, slope, intercept, max_change, min_val,
+                                alpha, max_bin, k_bin, n, k, num_bins, -1);
+}

@@ -1,0 +1,119 @@
+/*
+ * CPU oracle for the greedy sparse-recovery path (test infrastructure only).
+ *
+ * This library is the checker for the HIP implementation under gaga_amd/.
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load it.  It restates, in plain C, the algorithms of the reference's GPU
+ * path (GAGA/gaga/algorithms.cu, functions.cu, kernels.cu,
+ * GAGA/Matrices/{gen,smv,dct}_gpu.cu, results.cu) in two precisions:
+ *   orc32_*  float storage and float scalar logic (the reference's types);
+ *            inner products are accumulated in double and rounded once;
+ *   orc64_*  double everywhere (the FP64 oracle that errors are quoted against).
+ * The binning / support-selection primitives (orc_linear_binning,
+ * orc_find_support_set) follow the reference's float arithmetic exactly and
+ * are the bit-exact target for the device binning kernels.
+ *
+ * Parity pins (see tests/test_oracle.py):
+ *   philox           Random123 known-answer vectors
+ *   gen/smv/dct ops  numpy / scipy.sparse / scipy.fft fixtures (tests/golden)
+ *   binning          brute force on tiny inputs + top-k special case
+ *   CG steps         numpy.linalg.lstsq on a fixed support
+ *   algorithms       recovery of planted vectors; stop rules on synthetic
+ *                    residual histories
+ * Problem generators are this project's own (Philox4x32-10 + Box-Muller); the
+ * reference's cuRAND streams are not reproduced (parity unpinned there).
+ */
+#ifndef GAGA_ORACLE_H
+#define GAGA_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------- counter-based RNG (Salmon et al., SC'11) ---------------- */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+/* stream ids used by every generator (device generator uses the same ones) */
+enum { ORC_STREAM_A = 0, ORC_STREAM_SUPPORT = 1, ORC_STREAM_VALUES = 2,
+       ORC_STREAM_SMV_ROWS = 3, ORC_STREAM_SMV_VALS = 4, ORC_STREAM_DCT_ROWS = 5 };
+
+/* ---------------- problem generators ---------------- */
+/* A[i + j*lda] ~ N(0,1)/sqrt(m) for i<m, zero for m<=i<lda (column major). */
+void orc_gen_matrix(int m, int n, int lda, uint32_t seed, uint32_t inst, float *A);
+/* Uniform k-subset of [0,n) (Floyd's algorithm), returned ascending. */
+void orc_support(int n, int k, uint32_t seed, uint32_t inst, int *idx);
+/* x: zero except on orc_support(); dist 0 uniform(0,1), 1 binary +-1, 2 N(0,1). */
+void orc_sparse_vector(int n, int k, int dist, uint32_t seed, uint32_t inst, float *x);
+/* smv: p distinct rows per column (CSC with fixed p: rows[j*p+t]).
+ * ens 1 ones/sqrt(p), 2 +-1/sqrt(p), 3 ones.  Returns the number of empty rows. */
+int orc_smv_matrix(int m, int n, int p, int ens, uint32_t seed, uint32_t inst,
+                   int *rows, float *vals);
+/* dct: uniform m-subset of [0,n) row indices, ascending. */
+void orc_dct_rows(int m, int n, uint32_t seed, uint32_t inst, int *rows);
+
+/* ---------------- support selection (float-exact) ---------------- */
+/* kernels.cu:135-146 */
+void orc_linear_binning(const float *v, int n, int num_bins, int max_bin, float slope,
+                        float intercept, int *bins, int *counters);
+/* functions.cu:397-436.  State (*min_val, *alpha, *max_bin, *k_bin) is updated
+ * in place; returns `sum` (or -1 on the skip branch, where the reference
+ * returns an uninitialised value). */
+int orc_find_support_set(const float *v, int *bins, int *counters, float slope,
+                         float intercept, float max_change, float *min_val, float *alpha,
+                         int *max_bin, int *k_bin, int n, int k, int num_bins);
+
+/* ---------------- operators ---------------- */
+enum { ORC_ENS_GEN = 0, ORC_ENS_SMV = 1, ORC_ENS_DCT = 2 };
+enum { ORC_NIHT = 0, ORC_HTP = 1, ORC_CSMPSP = 2, ORC_CGIHT = 3 };
+
+typedef struct {
+  int ens, m, n;
+  /* gen */
+  const float *A; int lda;
+  /* smv: CSC */
+  const int *col_ptr; const int *rows; const float *vals; int nz;
+  /* dct */
+  const int *dct_rows;
+} orc_matrix;
+
+void orc_apply_A64(const orc_matrix *M, const double *x, double *y);   /* y = A x */
+void orc_apply_AT64(const orc_matrix *M, const double *r, double *g);  /* g = A^T r */
+/* orthonormal DCT-II / DCT-III of length n (power of two), double */
+void orc_dct2_ortho(int n, const double *x, double *X);
+void orc_dct3_ortho(int n, const double *X, double *x);
+
+/* ---------------- solvers ---------------- */
+typedef struct {
+  float tol;        /* stop when ||r||_2 <= tol */
+  int maxiter;
+  int num_bins;
+  int restart;      /* CGIHT restartFlag */
+  int threads;      /* OpenMP threads for the operators (<=0: all) */
+} orc_opts;
+
+typedef struct {
+  int iters;
+  int sum;              /* last support count */
+  int k_bin;
+  float conv_rate;      /* results.cu:365-378 form */
+  float conv_rate_given;/* gaga_gen.cu:735-742 form (user-problem path) */
+  float resid_hist[16]; /* residNorm_prev at exit */
+  int stop_reason;      /* 0 tol, 1 maxiter, 2 divergence, 3 diff, 4 fail, 5 cycle */
+} orc_result;
+
+/* x_out[n]: in = ignored (start is x=0), out = recovered vector.
+ * trace_kbin / trace_err (may be NULL): per outer iteration, length maxiter+1:
+ * entry 0 is the initial k_bin / ||y||, entry t the state after iteration t. */
+int orc32_solve(int alg, const orc_matrix *M, const float *y, int k, const orc_opts *o,
+                float *x_out, orc_result *res, int *trace_kbin, float *trace_err);
+int orc64_solve(int alg, const orc_matrix *M, const float *y, int k, const orc_opts *o,
+                double *x_out, orc_result *res, int *trace_kbin, double *trace_err);
+
+/* relative l1, l2, linf errors (results.cu:444-487) and TP,FP,TN,FN. */
+void orc_results(int n, int k, const float *x_true, const float *x_out, double norms[3],
+                 int support[4]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,241 @@
+"""ctypes wrapper over the CPU oracle (test infrastructure only).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module.  The product path (gaga_amd) never does.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+
+ALGS = {"NIHT": 0, "HTP": 1, "CSMPSP": 2, "CGIHT": 3}
+ENS = {"gen": 0, "smv": 1, "dct": 2}
+DIST = {"uniform": 0, "binary": 1, "gaussian": 2}
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(_LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+class Matrix(ctypes.Structure):
+    _fields_ = [
+        ("ens", ctypes.c_int), ("m", ctypes.c_int), ("n", ctypes.c_int),
+        ("A", ctypes.c_void_p), ("lda", ctypes.c_int),
+        ("col_ptr", ctypes.c_void_p), ("rows", ctypes.c_void_p), ("vals", ctypes.c_void_p),
+        ("nz", ctypes.c_int),
+        ("dct_rows", ctypes.c_void_p),
+    ]
+
+
+class Opts(ctypes.Structure):
+    _fields_ = [("tol", ctypes.c_float), ("maxiter", ctypes.c_int), ("num_bins", ctypes.c_int),
+                ("restart", ctypes.c_int), ("threads", ctypes.c_int)]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("iters", ctypes.c_int), ("sum", ctypes.c_int), ("k_bin", ctypes.c_int),
+                ("conv_rate", ctypes.c_float), ("conv_rate_given", ctypes.c_float),
+                ("resid_hist", ctypes.c_float * 16), ("stop_reason", ctypes.c_int)]
+
+
+_p = ctypes.c_void_p
+_i = ctypes.c_int
+_u = ctypes.c_uint32
+_f = ctypes.c_float
+
+
+def _declare(L):
+    L.orc_philox4x32_10.argtypes = [_p, _p, _p]
+    L.orc_gen_matrix.argtypes = [_i, _i, _i, _u, _u, _p]
+    L.orc_support.argtypes = [_i, _i, _u, _u, _p]
+    L.orc_sparse_vector.argtypes = [_i, _i, _i, _u, _u, _p]
+    L.orc_smv_matrix.argtypes = [_i, _i, _i, _i, _u, _u, _p, _p]
+    L.orc_smv_matrix.restype = _i
+    L.orc_dct_rows.argtypes = [_i, _i, _u, _u, _p]
+    L.orc_linear_binning.argtypes = [_p, _i, _i, _i, _f, _f, _p, _p]
+    L.orc_find_support_set.argtypes = [_p, _p, _p, _f, _f, _f, _p, _p, _p, _p, _i, _i, _i]
+    L.orc_find_support_set.restype = _i
+    L.orc_apply_A64.argtypes = [_p, _p, _p]
+    L.orc_apply_AT64.argtypes = [_p, _p, _p]
+    L.orc_dct2_ortho.argtypes = [_i, _p, _p]
+    L.orc_dct3_ortho.argtypes = [_i, _p, _p]
+    L.orc32_solve.argtypes = [_i, _p, _p, _i, _p, _p, _p, _p, _p]
+    L.orc64_solve.argtypes = [_i, _p, _p, _i, _p, _p, _p, _p, _p]
+    L.orc_results.argtypes = [_i, _i, _p, _p, _p, _p]
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+def philox(ctr, key):
+    c = np.asarray(ctr, dtype=np.uint32)
+    k = np.asarray(key, dtype=np.uint32)
+    out = np.zeros(4, dtype=np.uint32)
+    lib().orc_philox4x32_10(_ptr(c), _ptr(k), _ptr(out))
+    return out
+
+
+def gen_matrix(m, n, seed, inst, lda=None):
+    lda = m if lda is None else lda
+    A = np.empty((n, lda), dtype=np.float32)  # column j is row j of this array
+    lib().orc_gen_matrix(m, n, lda, seed, inst, _ptr(A))
+    return A
+
+
+def support(n, k, seed, inst):
+    idx = np.empty(max(k, 1), dtype=np.int32)
+    lib().orc_support(n, k, seed, inst, _ptr(idx))
+    return idx[:k]
+
+
+def sparse_vector(n, k, dist, seed, inst):
+    x = np.empty(n, dtype=np.float32)
+    lib().orc_sparse_vector(n, k, DIST.get(dist, dist), seed, inst, _ptr(x))
+    return x
+
+
+def smv_matrix(m, n, p, ens, seed, inst):
+    rows = np.empty(n * p, dtype=np.int32)
+    vals = np.empty(n * p, dtype=np.float32)
+    empty = lib().orc_smv_matrix(m, n, p, ens, seed, inst, _ptr(rows), _ptr(vals))
+    return rows, vals, empty
+
+
+def dct_rows(m, n, seed, inst):
+    rows = np.empty(m, dtype=np.int32)
+    lib().orc_dct_rows(m, n, seed, inst, _ptr(rows))
+    return rows
+
+
+def linear_binning(v, num_bins, max_bin, slope, intercept):
+    v = np.ascontiguousarray(v, dtype=np.float32)
+    bins = np.empty(v.size, dtype=np.int32)
+    counters = np.zeros(num_bins, dtype=np.int32)
+    lib().orc_linear_binning(_ptr(v), v.size, num_bins, max_bin, slope, intercept, _ptr(bins),
+                             _ptr(counters))
+    return bins, counters
+
+
+def find_support_set(v, k, num_bins, slope, intercept, max_change, min_val, alpha, max_bin, k_bin):
+    """One FindSupportSet call; returns (sum, bins, state dict)."""
+    v = np.ascontiguousarray(v, dtype=np.float32)
+    bins = np.zeros(v.size, dtype=np.int32)
+    counters = np.zeros(num_bins, dtype=np.int32)
+    mv = np.array([min_val], dtype=np.float32)
+    al = np.array([alpha], dtype=np.float32)
+    mb = np.array([max_bin], dtype=np.int32)
+    kb = np.array([k_bin], dtype=np.int32)
+    s = lib().orc_find_support_set(_ptr(v), _ptr(bins), _ptr(counters), slope, intercept,
+                                   max_change, _ptr(mv), _ptr(al), _ptr(mb), _ptr(kb), v.size, k,
+                                   num_bins)
+    return s, bins, dict(min_val=float(mv[0]), alpha=float(al[0]), max_bin=int(mb[0]),
+                         k_bin=int(kb[0]), counters=counters)
+
+
+class Problem:
+    """Holds the arrays an orc_matrix points to (keeps them alive)."""
+
+    def __init__(self, ens, m, n, A=None, col_ptr=None, rows=None, vals=None, dct_rows=None):
+        self.ens, self.m, self.n = ens, m, n
+        self.A = None if A is None else np.ascontiguousarray(A, dtype=np.float32)
+        self.col_ptr = None if col_ptr is None else np.ascontiguousarray(col_ptr, dtype=np.int32)
+        self.rows = None if rows is None else np.ascontiguousarray(rows, dtype=np.int32)
+        self.vals = None if vals is None else np.ascontiguousarray(vals, dtype=np.float32)
+        self.dct_rows = None if dct_rows is None else np.ascontiguousarray(dct_rows, dtype=np.int32)
+        M = Matrix()
+        M.ens, M.m, M.n = ENS[ens], m, n
+        if self.A is not None:
+            M.A = self.A.ctypes.data
+            M.lda = self.A.shape[1]
+        if self.col_ptr is not None:
+            M.col_ptr, M.rows, M.vals = self.col_ptr.ctypes.data, self.rows.ctypes.data, self.vals.ctypes.data
+            M.nz = self.rows.size
+        if self.dct_rows is not None:
+            M.dct_rows = self.dct_rows.ctypes.data
+        self.M = M
+
+    @classmethod
+    def gen(cls, A_cols, m=None):
+        """A_cols: array (n, lda) whose row j is column j of A (column-major A)."""
+        n, lda = A_cols.shape
+        return cls("gen", lda if m is None else m, n, A=A_cols)
+
+    @classmethod
+    def smv_fixed_p(cls, m, n, p, rows, vals):
+        col_ptr = np.arange(0, n * p + 1, p, dtype=np.int32)
+        return cls("smv", m, n, col_ptr=col_ptr, rows=rows, vals=vals)
+
+    def apply_A(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty(self.m, dtype=np.float64)
+        lib().orc_apply_A64(ctypes.byref(self.M), _ptr(x), _ptr(y))
+        return y
+
+    def apply_AT(self, r):
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        g = np.empty(self.n, dtype=np.float64)
+        lib().orc_apply_AT64(ctypes.byref(self.M), _ptr(r), _ptr(g))
+        return g
+
+
+def make_problem(ens, m, n, seed, inst, p=7, smv_ens=2, lda=None):
+    if ens == "gen":
+        return Problem.gen(gen_matrix(m, n, seed, inst, lda=lda), m=m)
+    if ens == "smv":
+        rows, vals, _ = smv_matrix(m, n, p, smv_ens, seed, inst)
+        return Problem.smv_fixed_p(m, n, p, rows, vals)
+    if ens == "dct":
+        return Problem("dct", m, n, dct_rows=dct_rows(m, n, seed, inst))
+    raise ValueError(ens)
+
+
+def solve(alg, prob, y, k, tol=1e-5, maxiter=None, num_bins=0, restart=0, precision=64,
+          trace=False):
+    """Run the oracle solver.  Returns (x, Result, trace_kbin, trace_err)."""
+    if maxiter is None:
+        maxiter = 300 if alg in ("HTP", "CSMPSP") else 5000
+    o = Opts(tol=tol, maxiter=maxiter, num_bins=num_bins, restart=restart, threads=0)
+    y = np.ascontiguousarray(y, dtype=np.float32)
+    res = Result()
+    tk = np.full(maxiter + 2, -1, dtype=np.int32) if trace else None
+    if precision == 64:
+        x = np.zeros(prob.n, dtype=np.float64)
+        te = np.zeros(maxiter + 2, dtype=np.float64) if trace else None
+        lib().orc64_solve(ALGS[alg], ctypes.byref(prob.M), _ptr(y), k, ctypes.byref(o), _ptr(x),
+                          ctypes.byref(res), _ptr(tk), _ptr(te))
+    else:
+        x = np.zeros(prob.n, dtype=np.float32)
+        te = np.zeros(maxiter + 2, dtype=np.float32) if trace else None
+        lib().orc32_solve(ALGS[alg], ctypes.byref(prob.M), _ptr(y), k, ctypes.byref(o), _ptr(x),
+                          ctypes.byref(res), _ptr(tk), _ptr(te))
+    if trace:
+        nt = res.iters + 1
+        return x, res, tk[:nt], te[:nt]
+    return x, res, None, None
+
+
+def results(x_true, x_out, k):
+    x_true = np.ascontiguousarray(x_true, dtype=np.float32)
+    x_out = np.ascontiguousarray(x_out, dtype=np.float32)
+    norms = np.zeros(3, dtype=np.float64)
+    sup = np.zeros(4, dtype=np.int32)
+    lib().orc_results(x_true.size, k, _ptr(x_true), _ptr(x_out), _ptr(norms), _ptr(sup))
+    return norms, sup

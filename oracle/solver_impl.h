@@ -1,0 +1,576 @@
+/*
+ * Precision-generic restatement of the reference's greedy algorithms.
+ * Included twice by oracle.c: once with REAL=float (PFX orc32_), once with
+ * REAL=double (PFX orc64_).  Test infrastructure only.
+ *
+ * Every function cites the reference lines it follows.  Deviations that are
+ * deliberate (documented in the project notes):
+ *   - checkSupportEvolution is evaluated race-free (kernels.cu:178-193 can
+ *     lose the flag across blocks);
+ *   - FindSupportSet's skip branch keeps the previous `sum` instead of
+ *     returning an uninitialised value (functions.cu:402-403, 435);
+ *   - inner products / norms are accumulated in double, then rounded to REAL.
+ */
+#define CAT2(a, b) a##b
+#define CAT(a, b) CAT2(a, b)
+#define F(name) CAT(PFX, name)
+
+/* ---------------- BLAS-1 (cuBLAS call sites in functions.cu) ---------------- */
+static REAL F(dot)(int n, const REAL *a, const REAL *b) {
+  double s = 0.0;
+  for (int i = 0; i < n; i++) s += (double)a[i] * (double)b[i];
+  return (REAL)s;
+}
+static REAL F(nrm2)(int n, const REAL *a) {
+  double s = 0.0;
+  for (int i = 0; i < n; i++) s += (double)a[i] * (double)a[i];
+  return (REAL)sqrt(s);
+}
+/* y += a*x (cublasSaxpy) */
+static void F(axpy)(int n, REAL a, const REAL *x, REAL *y) {
+  for (int i = 0; i < n; i++) y[i] = FMA(a, x[i], y[i]);
+}
+static void F(scal)(int n, REAL a, REAL *x) {
+  for (int i = 0; i < n; i++) x[i] = a * x[i];
+}
+/* MaxMagnitude (functions.cu:382-393): |v| at cublasIsamax's index. */
+static REAL F(maxmag)(int n, const REAL *v) {
+  REAL mx = 0;
+  for (int i = 0; i < n; i++) {
+    REAL a = FABS(v[i]);
+    if (a > mx) mx = a;
+  }
+  return mx;
+}
+/* threshold (kernels.cu:74-81) */
+static void F(threshold)(int n, REAL *v, const int *bins, int k_bin) {
+  for (int i = 0; i < n; i++)
+    if (bins[i] > k_bin) v[i] = 0;
+}
+/* zero_vector_float + threshold_one (kernels.cu:64-71, 100-105) */
+static void F(threshold_one)(int n, const REAL *src, REAL *dst, const int *bins, int k_bin) {
+  for (int i = 0; i < n; i++) dst[i] = (bins[i] <= k_bin) ? src[i] : (REAL)0;
+}
+/* max_list (functions.cu:216-226) */
+static REAL F(max_list)(const REAL *l, int N) {
+  REAL mx = l[0];
+  for (int j = 1; j < N; j++)
+    if (l[j] > mx) mx = l[j];
+  return mx;
+}
+
+/* ---------------- support selection ---------------- */
+/* LinearBinning (kernels.cu:135-146) in REAL arithmetic.  For REAL=float this
+ * is the exact float semantics: the `intercept*.000001` literal is a double,
+ * so that comparison happens in double; slope*(intercept-temp) has no
+ * contraction opportunity; the small-entry branch is fused (fma) as nvcc's
+ * default contraction produces. */
+static void F(linear_binning)(const REAL *v, int n, int num_bins, int max_bin, REAL slope,
+                              REAL intercept, int *bins, int *counters) {
+  (void)num_bins;
+  for (int i = 0; i < n; i++) {
+    REAL temp = FABS(v[i]);
+    if ((double)temp > (double)intercept * .000001) {
+      REAL b = slope * (intercept - temp);
+      b = FMAX((REAL)0, b);
+      bins[i] = (int)b;
+      if (bins[i] < max_bin) counters[bins[i]]++;
+    } else {
+      bins[i] = (int)FMA(slope, intercept, (REAL)1);
+    }
+  }
+}
+
+/* FindSupportSet (functions.cu:397-436). */
+static int F(find_support_set)(const REAL *v, int *bins, int *counters, REAL slope,
+                               REAL intercept, REAL max_change, REAL *min_val, REAL *alpha,
+                               int *max_bin, int *p_k_bin, int n, int k, int num_bins,
+                               int prev_sum) {
+  REAL minVal = *min_val;
+  int maxBin = *max_bin;
+  int k_bin = *p_k_bin;
+  int sum = prev_sum;
+  if (minVal > max_change) {
+    minVal = minVal - max_change;
+  } else {
+    memset(counters, 0, sizeof(int) * (size_t)num_bins);
+    F(linear_binning)(v, n, num_bins, maxBin, slope, intercept, bins, counters);
+    k_bin = 0;
+    sum = 0;
+    while ((sum < k) & (k_bin < maxBin)) {
+      sum = sum + counters[k_bin];
+      k_bin++;
+    }
+    k_bin = k_bin - 1;
+    if (sum < k) {
+      *alpha = *alpha * (REAL)0.5;
+      *max_bin = (int)(num_bins * (1 - *alpha));
+    }
+    minVal = intercept - (REAL)(k_bin + 1) / slope;
+  }
+  *min_val = minVal;
+  *p_k_bin = k_bin;
+  return sum;
+}
+
+/* ---------------- operators in REAL ---------------- */
+static void F(A)(const orc_matrix *M, const REAL *x, REAL *y) {
+#if REAL_IS_DOUBLE
+  orc_apply_A64(M, x, y);
+#else
+  double *xd = (double *)malloc(sizeof(double) * (size_t)M->n);
+  double *yd = (double *)malloc(sizeof(double) * (size_t)M->m);
+  for (int i = 0; i < M->n; i++) xd[i] = x[i];
+  orc_apply_A64(M, xd, yd);
+  for (int i = 0; i < M->m; i++) y[i] = (REAL)yd[i];
+  free(xd);
+  free(yd);
+#endif
+}
+static void F(AT)(const orc_matrix *M, const REAL *r, REAL *g) {
+#if REAL_IS_DOUBLE
+  orc_apply_AT64(M, r, g);
+#else
+  double *rd = (double *)malloc(sizeof(double) * (size_t)M->m);
+  double *gd = (double *)malloc(sizeof(double) * (size_t)M->n);
+  for (int i = 0; i < M->m; i++) rd[i] = r[i];
+  orc_apply_AT64(M, rd, gd);
+  for (int i = 0; i < M->n; i++) g[i] = (REAL)gd[i];
+  free(rd);
+  free(gd);
+#endif
+}
+
+/* ---------------- solver workspace ---------------- */
+typedef struct {
+  const orc_matrix *M;
+  int m, n, k, num_bins;
+  REAL *x, *thres, *grad, *grad_prev, *grad_prev_thres, *y, *resid, *resid_update;
+  int *bins, *bins_grad, *counters;
+  REAL residNorm_prev[16];
+} F(ws);
+
+static void F(push)(REAL *hist, REAL v) {
+  for (int j = 0; j < 15; j++) hist[j] = hist[j + 1];
+  hist[15] = v;
+}
+
+/* RestrictedSD_S_gen (functions.cu:659-745): returns maxChange. */
+static REAL F(restricted_sd)(F(ws) *w, int k_bin) {
+  int m = w->m, n = w->n;
+  memcpy(w->resid, w->y, sizeof(REAL) * (size_t)m);
+  F(A)(w->M, w->x, w->resid_update);
+  F(axpy)(m, (REAL)-1, w->resid_update, w->resid);
+  REAL err = F(nrm2)(m, w->resid);
+  F(push)(w->residNorm_prev, err);
+  F(AT)(w->M, w->resid, w->grad);
+  F(threshold_one)(n, w->grad, w->thres, w->bins, k_bin);
+  F(A)(w->M, w->thres, w->resid_update);
+  err = F(nrm2)(m, w->resid_update);
+  REAL mu = F(nrm2)(n, w->thres);
+  if (mu < 400 * err) {
+    mu = mu / err;
+    mu = mu * mu;
+  } else {
+    mu = 0;
+  }
+  F(axpy)(n, mu, w->grad, w->x);
+  mu = 2 * mu * F(maxmag)(n, w->grad);
+  return mu;
+}
+
+/* RestrictedCG_S_gen (functions.cu:922-1000).  grad: search direction,
+ * grad_prev: CG residual; hist: the CG history ring. */
+static void F(restricted_cg)(F(ws) *w, REAL *mu, int k_bin, REAL *hist) {
+  int m = w->m, n = w->n;
+  REAL alpha, beta;
+  F(A)(w->M, w->grad, w->resid);
+  alpha = F(dot)(m, w->resid, w->resid);
+  if (*mu < (1000 * alpha))
+    alpha = *mu / alpha;
+  else
+    alpha = 0;
+  F(axpy)(n, alpha, w->grad, w->x);
+  F(AT)(w->M, w->resid, w->thres);
+  F(threshold)(n, w->thres, w->bins, k_bin);
+  F(axpy)(n, -alpha, w->thres, w->grad_prev);
+  beta = *mu;
+  *mu = F(dot)(n, w->grad_prev, w->grad_prev);
+  if (*mu < (1000 * beta))
+    beta = *mu / beta;
+  else
+    beta = 0;
+  F(scal)(n, beta, w->grad);
+  F(axpy)(n, (REAL)1, w->grad_prev, w->grad);
+  F(A)(w->M, w->x, w->resid_update);
+  memcpy(w->resid, w->y, sizeof(REAL) * (size_t)m);
+  F(axpy)(m, (REAL)-1, w->resid_update, w->resid);
+  REAL err = F(dot)(m, w->resid, w->resid);
+  F(push)(hist, err);
+}
+
+/* RestrictedCGwithSupportEvolution_S_gen (functions.cu:1004-1102). */
+static REAL F(cg_supp_evolution)(F(ws) *w, REAL *mu, int beta_to_zero, int k_bin) {
+  int m = w->m, n = w->n;
+  REAL alpha, beta;
+  F(A)(w->M, w->x, w->resid_update);
+  memcpy(w->resid, w->y, sizeof(REAL) * (size_t)m);
+  F(axpy)(m, (REAL)-1, w->resid_update, w->resid);
+  REAL err = F(nrm2)(m, w->resid);
+  F(push)(w->residNorm_prev, err);
+  F(AT)(w->M, w->resid, w->grad);
+  F(threshold_one)(n, w->grad, w->thres, w->bins, k_bin);
+  if (beta_to_zero == 1) {
+    beta = 0;
+    *mu = F(dot)(n, w->thres, w->thres);
+  } else {
+    beta = *mu;
+    *mu = F(dot)(n, w->thres, w->thres);
+    if (*mu < (1000 * beta))
+      beta = *mu / beta;
+    else
+      beta = 0;
+  }
+  F(scal)(n, beta, w->grad_prev);
+  F(axpy)(n, (REAL)1, w->grad, w->grad_prev);
+  F(threshold_one)(n, w->grad_prev, w->grad_prev_thres, w->bins, k_bin);
+  F(A)(w->M, w->grad_prev_thres, w->resid_update);
+  alpha = F(dot)(m, w->resid_update, w->resid_update);
+  if (*mu < (1000 * alpha))
+    alpha = *mu / alpha;
+  else
+    alpha = 0;
+  F(axpy)(n, alpha, w->grad_prev, w->x);
+  alpha = 2 * alpha * F(maxmag)(n, w->grad_prev);
+  return alpha;
+}
+
+/* UnrestrictedCG_S_gen (functions.cu:1107-1220). */
+static REAL F(cg_unrestricted)(F(ws) *w, REAL *mu, int beta_to_zero, int k_bin) {
+  int m = w->m, n = w->n;
+  F(A)(w->M, w->x, w->resid_update);
+  F(axpy)(m, (REAL)-1, w->y, w->resid_update);
+  F(scal)(m, (REAL)-1, w->resid_update);
+  REAL err = F(nrm2)(m, w->resid_update);
+  F(push)(w->residNorm_prev, err);
+  F(AT)(w->M, w->resid_update, w->grad);
+  F(threshold_one)(n, w->grad, w->thres, w->bins, k_bin);
+  F(A)(w->M, w->grad_prev_thres, w->resid);
+  F(A)(w->M, w->thres, w->resid_update);
+  REAL beta;
+  if (beta_to_zero == 1) {
+    beta = 0;
+  } else {
+    REAL beta_num = F(dot)(m, w->resid, w->resid_update);
+    REAL beta_denom = F(dot)(m, w->resid, w->resid);
+    beta = -beta_num / beta_denom;
+  }
+  F(scal)(n, beta, w->grad_prev);
+  F(axpy)(n, (REAL)1, w->grad, w->grad_prev);
+  F(threshold_one)(n, w->grad_prev, w->grad_prev_thres, w->bins, k_bin);
+  F(scal)(m, beta, w->resid);
+  F(axpy)(m, (REAL)1, w->resid_update, w->resid);
+  REAL alpha_num = F(dot)(n, w->grad_prev_thres, w->thres);
+  REAL alpha_denom = F(dot)(m, w->resid, w->resid);
+  REAL alpha = alpha_num / alpha_denom;
+  F(axpy)(n, alpha, w->grad_prev, w->x);
+  *mu = F(dot)(n, w->thres, w->thres);
+  alpha = 2 * alpha * F(maxmag)(n, w->grad_prev);
+  return alpha;
+}
+
+/* Inner CG projection used by HTP and CSMPSP (algorithms.cu:2607-2669,
+ * 3548-3604).  Returns errCG (squared residual norm). */
+static REAL F(cg_projection)(F(ws) *w, int k_bin, REAL tolCG, REAL tol2, int fixed_diff) {
+  int m = w->m, n = w->n;
+  REAL hist[16], evo[16];
+  int iterCG = 0;
+  for (int j = 0; j < 16; j++) {
+    hist[j] = 0;
+    evo[j] = 1;
+  }
+  REAL diffCG = 1;
+  hist[15] = w->residNorm_prev[15];
+  memcpy(w->resid, w->y, sizeof(REAL) * (size_t)m);
+  F(A)(w->M, w->x, w->resid_update);
+  F(axpy)(m, (REAL)-1, w->resid_update, w->resid);
+  F(AT)(w->M, w->resid, w->grad);
+  F(threshold)(n, w->grad, w->bins, k_bin);
+  memcpy(w->grad_prev, w->grad, sizeof(REAL) * (size_t)n);
+  REAL mu = F(dot)(n, w->grad_prev, w->grad_prev);
+  REAL errCG = F(dot)(m, w->resid, w->resid);
+  hist[15] = errCG;
+  while ((errCG > tolCG) & (iterCG < 15) & (diffCG > tol2)) {
+    F(restricted_cg)(w, &mu, k_bin, hist);
+    errCG = hist[15];
+    iterCG++;
+    for (int j = 0; j < 15; j++) evo[j] = evo[j + 1];
+    evo[15] = hist[14] - hist[15];
+    diffCG = fixed_diff ? (REAL)10 : F(max_list)(evo, 16);
+  }
+  return errCG;
+}
+
+/* Stop-rule bookkeeping shared by all four loops. */
+typedef struct {
+  REAL err, err_start, diff;
+  REAL evo[16];
+  int fail, cycle;
+} F(stop);
+
+static int F(should_continue)(const F(stop) *s, REAL tol, int iter, int maxiter, int *reason) {
+  if (!(s->err > tol)) { *reason = 0; return 0; }
+  if (!(iter < maxiter)) { *reason = 1; return 0; }
+  if (!(s->err < (100 * s->err_start))) { *reason = 2; return 0; }
+  if (!((double)s->diff > .01 * (double)tol)) { *reason = 3; return 0; }
+  if (s->fail != 0) { *reason = 4; return 0; }
+  if (!s->cycle) { *reason = 5; return 0; }
+  return 1;
+}
+
+/* algorithms.cu:1405-1421 (NIHT/CGIHT: fail after 749) and 2680-2697 (HTP/CSMPSP: after 125). */
+static void F(stop_update)(F(stop) *s, const REAL *prev, int iter, int fail_after) {
+  for (int j = 0; j < 15; j++) s->evo[j] = s->evo[j + 1];
+  s->evo[15] = prev[14] - prev[15];
+  s->diff = F(max_list)(s->evo, 16);
+  for (int j = 0; j < 15; j++)
+    if ((prev[15] == prev[j]) && (iter > 4)) s->cycle = 0;
+  if (iter > fail_after) {
+    REAL root = (REAL)(1 / 15.0f);
+    REAL rate = prev[15] / prev[0];
+    rate = POW(rate, root);
+    if (rate > (REAL)0.999f) s->fail = 1;
+  }
+}
+
+static void F(trace)(int *tk, REAL *te, int idx, int k_bin, REAL e) {
+  if (tk) tk[idx] = k_bin;
+  if (te) te[idx] = e;
+}
+
+int F(solve)(int alg, const orc_matrix *M, const float *y_in, int k, const orc_opts *o,
+             REAL *x_out, orc_result *res, int *trace_kbin, REAL *trace_err) {
+  int m = M->m, n = M->n;
+  int num_bins = o->num_bins > 0 ? o->num_bins : (n / 20 > 1000 ? n / 20 : 1000);
+  REAL tol = (REAL)o->tol;
+  int maxiter = o->maxiter;
+  F(ws) W;
+  F(ws) *w = &W;
+  memset(w, 0, sizeof(W));
+  w->M = M;
+  w->m = m;
+  w->n = n;
+  w->k = k;
+  w->num_bins = num_bins;
+  size_t nn = (size_t)n, mm = (size_t)m;
+  w->x = x_out;
+  w->thres = (REAL *)calloc(nn, sizeof(REAL));
+  w->grad = (REAL *)calloc(nn, sizeof(REAL));
+  w->grad_prev = (REAL *)calloc(nn, sizeof(REAL));
+  w->grad_prev_thres = (REAL *)calloc(nn, sizeof(REAL));
+  w->y = (REAL *)calloc(mm, sizeof(REAL));
+  w->resid = (REAL *)calloc(mm, sizeof(REAL));
+  w->resid_update = (REAL *)calloc(mm, sizeof(REAL));
+  w->bins = (int *)calloc(nn, sizeof(int));
+  w->bins_grad = (int *)calloc(nn, sizeof(int));
+  w->counters = (int *)calloc((size_t)num_bins, sizeof(int));
+  for (int i = 0; i < m; i++) w->y[i] = (REAL)y_in[i];
+  for (int i = 0; i < n; i++) w->x[i] = 0;
+
+  int iter = 0, k_bin = 0, k_bin_grad = 0, sum = 0;
+  REAL alpha = (REAL)0.25f;
+  int MaxBin = (int)(num_bins * (1 - alpha));
+  REAL minVal = 0;
+  REAL maxChange = (alg == ORC_NIHT || alg == ORC_CGIHT) ? (REAL)1 : (REAL)0;
+  REAL mu = 0;
+  F(stop) S;
+  memcpy(w->resid, w->y, sizeof(REAL) * mm);
+  S.err = F(nrm2)(m, w->resid);
+  S.err_start = S.err;
+  S.diff = 1;
+  for (int j = 0; j < 16; j++) {
+    w->residNorm_prev[j] = 0;
+    S.evo[j] = 1;
+  }
+  S.fail = 0;
+  S.cycle = 1;
+  int reason = 0;
+  REAL tolCG = tol * tol;
+  REAL tol2 = 10 * tolCG;
+
+  /* Set the linear bins (algorithms.cu:1340-1358 and equivalents). */
+  F(AT)(M, w->y, w->x);
+  REAL max_value = F(maxmag)(n, w->x);
+  if (max_value == 0) { /* y == 0 (or y orthogonal to every column): x = 0 */
+    res->iters = 0;
+    res->stop_reason = 0;
+    goto done;
+  }
+  REAL slope = ((REAL)(num_bins - 1) / (max_value));
+  sum = F(find_support_set)(w->x, w->bins, w->counters, slope, max_value,
+                            (alg == ORC_HTP || alg == ORC_CSMPSP) ? max_value : maxChange,
+                            &minVal, &alpha, &MaxBin, &k_bin, n, k, num_bins, sum);
+  F(threshold)(n, w->x, w->bins, k_bin);
+  F(trace)(trace_kbin, trace_err, 0, k_bin, S.err);
+
+  if (alg == ORC_NIHT) {
+    /* NIHT_S_gen (algorithms.cu:1274-1441) */
+    while (F(should_continue)(&S, tol, iter, maxiter, &reason)) {
+      maxChange = F(restricted_sd)(w, k_bin);
+      if (minVal <= maxChange) {
+        max_value = F(maxmag)(n, w->x);
+        slope = ((REAL)(num_bins - 1) / (max_value));
+      }
+      sum = F(find_support_set)(w->x, w->bins, w->counters, slope, max_value, maxChange,
+                                &minVal, &alpha, &MaxBin, &k_bin, n, k, num_bins, sum);
+      F(threshold)(n, w->x, w->bins, k_bin);
+      S.err = w->residNorm_prev[15];
+      F(stop_update)(&S, w->residNorm_prev, iter, 749);
+      iter++;
+      F(trace)(trace_kbin, trace_err, iter, k_bin, w->residNorm_prev[15]);
+    }
+  } else if (alg == ORC_HTP) {
+    /* HTP_S_gen (algorithms.cu:2491-2716) */
+    while (F(should_continue)(&S, tol, iter, maxiter, &reason)) {
+      maxChange = F(restricted_sd)(w, k_bin);
+      if (minVal <= maxChange) {
+        max_value = F(maxmag)(n, w->x);
+        slope = ((REAL)(num_bins - 1) / (max_value));
+      }
+      sum = F(find_support_set)(w->x, w->bins, w->counters, slope, max_value, maxChange,
+                                &minVal, &alpha, &MaxBin, &k_bin, n, k, num_bins, sum);
+      F(threshold)(n, w->x, w->bins, k_bin);
+      REAL errCG = F(cg_projection)(w, k_bin, tolCG, tol2, 0);
+      S.err = SQRT(errCG);
+      F(push)(w->residNorm_prev, S.err);
+      F(stop_update)(&S, w->residNorm_prev, iter, 125);
+      iter++;
+      F(trace)(trace_kbin, trace_err, iter, k_bin, w->residNorm_prev[15]);
+    }
+  } else if (alg == ORC_CSMPSP) {
+    /* CSMPSP_S_gen (algorithms.cu:3372-3674) */
+    REAL errCG = F(cg_projection)(w, k_bin, tolCG, tol2, 0);
+    S.err = SQRT(errCG);
+    w->residNorm_prev[15] = S.err;
+    iter++;
+    F(trace)(trace_kbin, trace_err, iter, k_bin, S.err);
+    while (F(should_continue)(&S, tol, iter, maxiter, &reason)) {
+      F(AT)(M, w->resid, w->grad);
+      max_value = F(maxmag)(n, w->grad);
+      minVal = 0;
+      slope = ((REAL)(num_bins - 1) / (max_value));
+      sum = F(find_support_set)(w->grad, w->bins_grad, w->counters, slope, max_value, max_value,
+                                &minVal, &alpha, &MaxBin, &k_bin_grad, n, k, num_bins, sum);
+      /* joinSupports (kernels.cu:162-172) */
+      for (int i = 0; i < n; i++)
+        if (w->bins_grad[i] <= k_bin_grad && w->bins[i] > k_bin) w->bins[i] = k_bin;
+      F(cg_projection)(w, k_bin, tolCG, tol2, 1);
+      max_value = F(maxmag)(n, w->x);
+      minVal = 0;
+      slope = ((REAL)(num_bins - 1) / (max_value));
+      sum = F(find_support_set)(w->x, w->bins, w->counters, slope, max_value, max_value,
+                                &minVal, &alpha, &MaxBin, &k_bin, n, k, num_bins, sum);
+      F(threshold)(n, w->x, w->bins, k_bin);
+      memcpy(w->resid, w->y, sizeof(REAL) * mm);
+      F(A)(M, w->x, w->resid_update);
+      F(axpy)(m, (REAL)-1, w->resid_update, w->resid);
+      S.err = F(nrm2)(m, w->resid);
+      F(push)(w->residNorm_prev, S.err);
+      F(stop_update)(&S, w->residNorm_prev, iter, 125);
+      iter++;
+      F(trace)(trace_kbin, trace_err, iter, k_bin, S.err);
+    }
+  } else if (alg == ORC_CGIHT) {
+    /* CGIHT_S_gen (algorithms.cu:3689-3942) */
+    F(A)(M, w->x, w->resid_update);
+    F(axpy)(m, (REAL)-1, w->resid_update, w->resid);
+    S.err = F(nrm2)(m, w->resid);
+    w->residNorm_prev[15] = S.err;
+    F(AT)(M, w->resid, w->grad_prev);
+    F(threshold_one)(n, w->grad_prev, w->grad_prev_thres, w->bins, k_bin);
+    for (int j = 0; j < 15; j++) S.evo[j] = S.evo[j + 1];
+    S.evo[15] = w->residNorm_prev[14] - w->residNorm_prev[15];
+    int beta_to_zero = 1, supp_change = 1, k_bin_prev = 0;
+    while (F(should_continue)(&S, tol, iter, maxiter, &reason)) {
+      if (o->restart || !supp_change)
+        maxChange = F(cg_supp_evolution)(w, &mu, beta_to_zero, k_bin);
+      else
+        maxChange = F(cg_unrestricted)(w, &mu, beta_to_zero, k_bin);
+      beta_to_zero = 0;
+      if (minVal <= maxChange) {
+        max_value = F(maxmag)(n, w->x);
+        slope = ((REAL)(num_bins - 1) / (max_value));
+        memcpy(w->bins_grad, w->bins, sizeof(int) * nn);
+        k_bin_prev = k_bin;
+        sum = F(find_support_set)(w->x, w->bins, w->counters, slope, max_value, maxChange,
+                                  &minVal, &alpha, &MaxBin, &k_bin, n, k, num_bins, sum);
+        /* checkSupportEvolution (kernels.cu:178-193), race-free */
+        int changed = 0;
+        for (int i = 0; i < n; i++)
+          if (w->bins_grad[i] <= k_bin_prev && w->bins[i] > k_bin) { changed = 1; break; }
+        supp_change = changed;
+        if (changed) {
+          if (o->restart)
+            beta_to_zero = 1;
+          else
+            F(threshold_one)(n, w->grad_prev, w->grad_prev_thres, w->bins, k_bin);
+        }
+      } else {
+        supp_change = 0;
+        minVal = minVal - maxChange;
+      }
+      F(threshold)(n, w->x, w->bins, k_bin);
+      S.err = w->residNorm_prev[15];
+      F(stop_update)(&S, w->residNorm_prev, iter, 749);
+      iter++;
+      F(trace)(trace_kbin, trace_err, iter, k_bin, w->residNorm_prev[15]);
+    }
+  }
+  res->iters = iter;
+  res->stop_reason = reason;
+
+done:
+  res->sum = sum;
+  res->k_bin = k_bin;
+  for (int j = 0; j < 16; j++) res->resid_hist[j] = (float)w->residNorm_prev[j];
+  {
+    /* results.cu:365-378 */
+    float convRate, root;
+    int temp = iter < 16 ? iter : 16;
+    if (temp <= 1) {
+      convRate = 0.0f;
+    } else {
+      temp = temp - 1;
+      root = 1 / (float)temp;
+      temp = 15 - temp;
+      convRate = (float)(w->residNorm_prev[15] / w->residNorm_prev[temp]);
+      convRate = powf(convRate, root);
+    }
+    res->conv_rate = convRate;
+    /* gaga_gen.cu:735-742 (convRateNum = 16) */
+    temp = iter < 16 ? iter : 16;
+    if (temp == 0) {
+      res->conv_rate_given = 0.0f;
+    } else {
+      root = 1 / (float)temp;
+      temp = 16 - temp;
+      convRate = (float)(w->residNorm_prev[15] / w->residNorm_prev[temp]);
+      res->conv_rate_given = powf(convRate, root);
+    }
+  }
+  free(w->thres);
+  free(w->grad);
+  free(w->grad_prev);
+  free(w->grad_prev_thres);
+  free(w->y);
+  free(w->resid);
+  free(w->resid_update);
+  free(w->bins);
+  free(w->bins_grad);
+  free(w->counters);
+  return 0;
+}
+
+#undef F
+#undef CAT
+#undef CAT2
