@@ -16,9 +16,9 @@ import warnings
 import numpy as np
 
 from . import _capi
-from ._capi import Plan, GagaError, load  # noqa: F401
+from ._capi import Plan, Batch, GagaError, load  # noqa: F401
 
-__all__ = ["cs", "options", "Plan", "GagaError", "load", "ALGORITHMS", "ENSEMBLES"]
+__all__ = ["cs", "options", "Plan", "Batch", "GagaError", "load", "ALGORITHMS", "ENSEMBLES"]
 
 ALGORITHMS = ("NIHT", "HTP", "CSMPSP", "CGIHT")
 ENSEMBLES = ("gen", "smv", "dct")

@@ -5,13 +5,15 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libgaga_amd.so")
+LIB_PATH = os.environ.get("XCS_LIB_PATH", os.path.join(_HERE, "libgaga_amd.so"))
 
 # every symbol include/gaga_amd.h declares
 EXPORTS = (
     "xcs_options_default", "xcs_cs_generate", "xcs_cs_gen", "xcs_cs_smv", "xcs_cs_dct",
     "xcs_plan_create", "xcs_plan_solve", "xcs_plan_results", "xcs_plan_trace",
-    "xcs_plan_destroy", "xcs_find_support_set", "xcs_apply", "xcs_generate_gen",
+    "xcs_plan_destroy", "xcs_plan_bytes", "xcs_plan_apply_timed", "xcs_batch_create", "xcs_batch_bytes", "xcs_batch_solve", "xcs_batch_results",
+    "xcs_batch_x", "xcs_batch_info", "xcs_batch_destroy", "xcs_batch_phase_ticks",
+    "xcs_find_support_set", "xcs_apply", "xcs_generate_gen",
     "xcs_last_error", "xcs_version",
 )
 
@@ -60,6 +62,17 @@ def load():
         L.xcs_plan_trace.argtypes = [_P, _P, _P, _I]
         L.xcs_plan_destroy.argtypes = [_P]
         L.xcs_plan_destroy.restype = None
+        L.xcs_plan_bytes.argtypes = [_P, _P]
+        L.xcs_plan_apply_timed.argtypes = [_P, _I, _I, _P, _P]
+        L.xcs_batch_bytes.argtypes = [_P, _P]
+        L.xcs_batch_create.argtypes = [_C, _I, _P, _P, _P, _P, _P, _P, _P, _P]
+        L.xcs_batch_solve.argtypes = [_P, _P]
+        L.xcs_batch_results.argtypes = [_P, _P, _P, _P, _P]
+        L.xcs_batch_x.argtypes = [_P, _I, _P, _P]
+        L.xcs_batch_info.argtypes = [_P, _P, _P, _P, _P]
+        L.xcs_batch_destroy.argtypes = [_P]
+        L.xcs_batch_destroy.restype = None
+        L.xcs_batch_phase_ticks.argtypes = [_P, _P]
         L.xcs_find_support_set.argtypes = [_P, _I, _I, _I, _F, _F, _F, _P, _P, _P, _P, _P, _P]
         L.xcs_apply.argtypes = [_C, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P]
         L.xcs_generate_gen.argtypes = [_I, _I, ctypes.c_uint, ctypes.c_uint, _P]
@@ -199,6 +212,18 @@ class Plan:
         return dict(norms=norms, iters=it.value, support=sup, conv_rate=rate.value,
                     stop_reason=reason.value, x=x, x_true=xt)
 
+    def bytes(self):
+        b = ctypes.c_double(0)
+        _ck(load().xcs_plan_bytes(self._h, ctypes.byref(b)))
+        return b.value
+
+    def apply_timed(self, transpose, reps):
+        ms = ctypes.c_float(0)
+        b = ctypes.c_double(0)
+        _ck(load().xcs_plan_apply_timed(self._h, int(transpose), int(reps), ctypes.byref(ms),
+                                        ctypes.byref(b)))
+        return ms.value, b.value
+
     def trace(self, length):
         kb = np.zeros(length, np.int32)
         er = np.zeros(length, np.float32)
@@ -208,6 +233,83 @@ class Plan:
     def close(self):
         if self._h:
             load().xcs_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+ENS_CODE = {"gen": 0, "smv": 1, "dct": 2}
+
+
+class Batch:
+    """Many independent instances solved in one persistent launch (xcs_batch_*).
+
+    specs: list of dicts with keys ens, k, m, n and optionally p, instance.
+    """
+
+    def __init__(self, alg, specs, opts=None):
+        from . import _apply_options
+        self.alg = alg
+        self.count = len(specs)
+        ens = np.array([ENS_CODE[s["ens"]] for s in specs], np.int32)
+        k = np.array([s["k"] for s in specs], np.int32)
+        m = np.array([s["m"] for s in specs], np.int32)
+        n = np.array([s["n"] for s in specs], np.int32)
+        p = np.array([s.get("p", 0) for s in specs], np.int32)
+        inst = np.array([s.get("instance", i) for i, s in enumerate(specs)], np.uint32)
+        self.n = n
+        first = specs[0]["ens"]
+        o = _apply_options(alg, first, int(n.max()), opts)
+        if any(s["ens"] == "smv" for s in specs) and not any(
+                name == "matrixEnsemble" for name, _ in (opts or [])):
+            o.matrix_ensemble = 2
+        h = ctypes.c_void_p()
+        _ck(load().xcs_batch_create(alg.encode(), self.count, _p(ens), _p(k), _p(m), _p(n), _p(p),
+                                    _p(inst), ctypes.byref(o), ctypes.byref(h)))
+        self._h = h
+
+    def solve(self):
+        ms = ctypes.c_float(0)
+        _ck(load().xcs_batch_solve(self._h, ctypes.byref(ms)))
+        return ms.value
+
+    def results(self):
+        norms = np.zeros((self.count, 3), np.float32)
+        it = np.zeros(self.count, np.int32)
+        reason = np.zeros(self.count, np.int32)
+        rate = np.zeros(self.count, np.float32)
+        _ck(load().xcs_batch_results(self._h, _p(norms), _p(it), _p(reason), _p(rate)))
+        return dict(norms=norms, iters=it, stop_reason=reason, conv_rate=rate)
+
+    def x(self, i):
+        x = np.zeros(int(self.n[i]), np.float32)
+        xt = np.zeros(int(self.n[i]), np.float32)
+        _ck(load().xcs_batch_x(self._h, i, _p(x), _p(xt)))
+        return x, xt
+
+    def bytes(self):
+        b = ctypes.c_double(0)
+        _ck(load().xcs_batch_bytes(self._h, ctypes.byref(b)))
+        return b.value
+
+    def info(self):
+        t, c, b, g = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_float()
+        _ck(load().xcs_batch_info(self._h, ctypes.byref(t), ctypes.byref(c), ctypes.byref(b),
+                                  ctypes.byref(g)))
+        return dict(teams=t.value, team_size=c.value, block=b.value, gen_ms=g.value)
+
+    def phase_ticks(self):
+        out = np.zeros(32, np.uint64)
+        _ck(load().xcs_batch_phase_ticks(self._h, _p(out)))
+        return out
+
+    def close(self):
+        if self._h:
+            load().xcs_batch_destroy(self._h)
             self._h = None
 
     def __del__(self):

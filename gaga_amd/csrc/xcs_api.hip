@@ -40,40 +40,78 @@ static int set_err(int code, const char *fmt, ...) {
   } while (0)
 
 // ----------------------------------------------------------------- kernels --
+// One instance, the whole (cooperatively launched) grid is its team.
+// The descriptor is copied to LDS: its fields then cannot alias the global
+// vectors the solver writes, so the compiler keeps them in registers.
 __global__ void __launch_bounds__(1024) solve_coop_kernel(const XProblem *P, int budget,
-                                                          double *red) {
+                                                          double *red, unsigned *bar) {
   __shared__ Lds L;
-  Team t{(int)blockIdx.x, (int)gridDim.x, red, 0};
-  solve_instance(t, L, *P, budget);
+  __shared__ XProblem sP;
+  if (threadIdx.x == 0) sP = *P;
+  __syncthreads();
+  Team t{(int)blockIdx.x, (int)gridDim.x, red, 0, bar};
+  solve_instance(t, L, sP, budget);
 }
 
-// Batched form: one workgroup per instance, instances taken from a queue.
+// Team membership for the batched engine: c workgroups per team.  Workgroups
+// are dispatched round-robin over the 8 XCDs (XCD = blockIdx % 8), so when the
+// grid allows it a team is formed from workgroups of one XCD (shared L2).
+__device__ __forceinline__ void batch_team(int c, int *team_id, int *rank) {
+  const int b = blockIdx.x, nb = gridDim.x;
+  if (c == 1) {
+    *team_id = b;
+    *rank = 0;
+  } else if (nb % (8 * c) == 0) {
+    const int xcd = b % 8, local = b / 8;
+    *team_id = (local / c) * 8 + xcd;
+    *rank = local % c;
+  } else {
+    *team_id = b / c;
+    *rank = b % c;
+  }
+}
+
+// Batched persistent form: teams of c workgroups take instances from a
+// queue and solve each to completion with team-scoped barriers only.
 __global__ void __launch_bounds__(1024) solve_batch_kernel(const XProblem *Ps, int count,
-                                                           int *queue, int budget) {
+                                                           int *queue, XProblem *teamP,
+                                                           const XWork *works, int *slots,
+                                                           double *red, unsigned *bar, int c,
+                                                           int budget) {
   __shared__ Lds L;
-  __shared__ int next;
-  Team t{0, 1, nullptr, 0};
+  __shared__ XProblem sP;
+  int team_id, rank;
+  batch_team(c, &team_id, &rank);
+  Team t{rank, c, red + (size_t)team_id * 2 * RED_SLOTS * c, 0, bar + (size_t)team_id * BAR_WORDS};
   while (true) {
-    if (threadIdx.x == 0) next = atomicAdd(queue, 1);
-    __syncthreads();
-    const int idx = next;
-    __syncthreads();
+    if (rank == 0 && threadIdx.x == 0) slots[team_id] = atomicAdd(queue, 1);
+    t.sync();
+    const int idx = slots[team_id];
     if (idx >= count) break;
-    solve_instance(t, L, Ps[idx], budget);
+    if (threadIdx.x == 0) {
+      sP = Ps[idx];
+      attach_work(sP, works[team_id]);
+    }
     __syncthreads();
+    solve_instance(t, L, sP, budget);
+    t.sync();
   }
 }
 
 // y = A x_S (mode 0) or g = A^T r (mode 1) with the team operators.
 __global__ void __launch_bounds__(1024) apply_coop_kernel(const XProblem *P, int transpose,
                                                           const float *in, float *out,
-                                                          int cnt, double *red) {
+                                                          int cnt, double *red, unsigned *bar) {
   __shared__ Lds L;
-  Team t{(int)blockIdx.x, (int)gridDim.x, red, 0};
+  __shared__ XProblem sP;
+  if (threadIdx.x == 0) sP = *P;
+  __syncthreads();
+  Team t{(int)blockIdx.x, (int)gridDim.x, red, 0, bar};
   if (transpose) {
-    op_AT(t, L, *P, in, [&](int j, float g) { out[j] = g; });
+    XG(float) o = (XG(float))out;
+    op_AT(t, L, sP, (XG(const float))in, [&](int j, float g) { o[j] = g; });
   } else {
-    op_A(t, L, *P, in, out, cnt);
+    op_A(t, L, sP, (XG(const float))in, (XG(float))out, cnt);
   }
 }
 
@@ -81,9 +119,12 @@ __global__ void __launch_bounds__(1024) apply_coop_kernel(const XProblem *P, int
 __global__ void __launch_bounds__(1024) find_support_coop_kernel(const XProblem *P,
                                                                  const float *v, XState *S_io,
                                                                  float slope, float intercept,
-                                                                 double *red) {
+                                                                 double *red, unsigned *bar) {
   __shared__ Lds L;
-  Team t{(int)blockIdx.x, (int)gridDim.x, red, 0};
+  __shared__ XProblem sP;
+  if (threadIdx.x == 0) sP = *P;
+  __syncthreads();
+  Team t{(int)blockIdx.x, (int)gridDim.x, red, 0, bar};
   XState S = *S_io;
   if (S.min_val > S.max_change) {
     if (t.rank == 0 && threadIdx.x == 0) {
@@ -92,10 +133,10 @@ __global__ void __launch_bounds__(1024) find_support_coop_kernel(const XProblem 
     }
     return;
   }
-  zero_counters(t, *P);
+  zero_counters(t, sP);
   t.sync();
   int kb;
-  find_support_bin(t, L, *P, v, P->bins, S, slope, intercept, &kb);
+  find_support_bin(t, L, sP, (XG(const float))v, sP.bins, S, slope, intercept, &kb);
   S.k_bin = kb;
   if (t.rank == 0 && threadIdx.x == 0) *S_io = S;
 }
@@ -235,6 +276,7 @@ struct xcs_plan {
   XProblem *d = nullptr;
   int team = 1, block = 256, mode = 0;
   double *red = nullptr;
+  unsigned *bar = nullptr;
   std::vector<void *> allocs;
   std::vector<float> x_true;
   int k_planted = 0;
@@ -338,12 +380,13 @@ static int plan_alloc(xcs_plan *pl, int alg, int ens, int k, int m, int n, int p
     hipLaunchKernelGGL(dct_tables_kernel, dim3(std::min(4096, (n + 255) / 256)), dim3(256), 0,
                        pl->stream, tw, n);
     HIP_TRY(hipGetLastError());
-    P.twiddle = tw;
+    xset(P.twiddle, tw);
   }
   XCS_TRY(pl->alloc(&P.st, 1));
   XCS_TRY(pl->alloc(&P.trace_err, (size_t)P.maxiter + 2));
   XCS_TRY(pl->alloc(&P.trace_kbin, (size_t)P.maxiter + 2));
   XCS_TRY(pl->alloc(&pl->red, (size_t)2 * RED_SLOTS * (pl->team + 1)));
+  XCS_TRY(pl->alloc(&pl->bar, (size_t)BAR_WORDS));
   XCS_TRY(pl->alloc(&pl->d, 1));
   return XCS_OK;
 }
@@ -354,8 +397,9 @@ static int plan_upload_desc(xcs_plan *pl) {
 }
 
 static int launch_apply(xcs_plan *pl, int transpose, const float *in, float *out, int cnt) {
+  HIP_TRY(hipMemsetAsync(pl->bar, 0, sizeof(unsigned) * BAR_WORDS, pl->stream));
   void *args[] = {(void *)&pl->d, (void *)&transpose, (void *)&in, (void *)&out, (void *)&cnt,
-                  (void *)&pl->red};
+                  (void *)&pl->red, (void *)&pl->bar};
   HIP_TRY(hipLaunchCooperativeKernel((const void *)apply_coop_kernel, dim3(pl->team),
                                      dim3(pl->block), args, 0, pl->stream));
   return XCS_OK;
@@ -406,12 +450,12 @@ static int plan_set_smv(xcs_plan *pl, const std::vector<int> &col_ptr, const std
   HIP_TRY(hipMemcpyAsync(d_c, cols.data(), sizeof(int) * nz, hipMemcpyHostToDevice, pl->stream));
   HIP_TRY(hipMemcpyAsync(d_rv, rvals.data(), sizeof(float) * nz, hipMemcpyHostToDevice, pl->stream));
   HIP_TRY(hipStreamSynchronize(pl->stream));
-  P.col_ptr = d_cp;
-  P.rows = d_r;
-  P.vals = d_v;
-  P.row_ptr = d_rp;
-  P.cols = d_c;
-  P.rvals = d_rv;
+  xset(P.col_ptr, d_cp);
+  xset(P.rows, d_r);
+  xset(P.vals, d_v);
+  xset(P.row_ptr, d_rp);
+  xset(P.cols, d_c);
+  xset(P.rvals, d_rv);
   P.nz = nz;
   return XCS_OK;
 }
@@ -451,7 +495,7 @@ static int plan_generate(xcs_plan *pl, int alg, int ens, int k, int m, int n, in
   XProblem &P = pl->h;
   float *y;
   XCS_TRY(pl->alloc(&y, (size_t)P.lda));
-  P.y = y;
+  xset(P.y, y);
   if (ens == ENS_GEN) {
     float *A;
     XCS_TRY(pl->alloc(&A, (size_t)P.lda * n));
@@ -460,7 +504,7 @@ static int plan_generate(xcs_plan *pl, int alg, int ens, int k, int m, int n, in
     hipLaunchKernelGGL(gen_matrix_kernel, dim3(blocks), dim3(256), 0, pl->stream, A, m, n, P.lda,
                        seed, inst);
     HIP_TRY(hipGetLastError());
-    P.A = A;
+    xset(P.A, A);
   } else if (ens == ENS_SMV) {
     csc_ptr.resize((size_t)n + 1);
     for (int j = 0; j <= n; j++) csc_ptr[j] = j * p;
@@ -471,7 +515,7 @@ static int plan_generate(xcs_plan *pl, int alg, int ens, int k, int m, int n, in
     int *d_rows;
     XCS_TRY(pl->alloc(&d_rows, (size_t)m));
     HIP_TRY(hipMemcpyAsync(d_rows, rows.data(), sizeof(int) * m, hipMemcpyHostToDevice, pl->stream));
-    P.dct_rows = d_rows;
+    xset(P.dct_rows, d_rows);
   }
   std::vector<int> supp;
   host_sparse_vector(n, k, o->vec_distribution, seed, inst, pl->x_true, supp);
@@ -492,6 +536,7 @@ static int plan_run(xcs_plan *pl, float *ms) {
   // reset: x = 0 and a fresh state (inputs stay resident)
   HIP_TRY(hipMemsetAsync(P.x, 0, sizeof(float) * P.n, pl->stream));
   HIP_TRY(hipMemsetAsync(P.st, 0, sizeof(XState), pl->stream));
+  HIP_TRY(hipMemsetAsync(pl->bar, 0, sizeof(unsigned) * BAR_WORDS, pl->stream));
   XCS_TRY(plan_upload_desc(pl));
   hipEvent_t e0, e1;
   HIP_TRY(hipEventCreate(&e0));
@@ -501,7 +546,7 @@ static int plan_run(xcs_plan *pl, float *ms) {
     // plain form: one outer iteration per launch, host reads the stop flag
     int budget = 1;
     for (int launch = 0; launch < P.maxiter + 4; launch++) {
-      void *args[] = {(void *)&pl->d, (void *)&budget, (void *)&pl->red};
+      void *args[] = {(void *)&pl->d, (void *)&budget, (void *)&pl->red, (void *)&pl->bar};
       HIP_TRY(hipLaunchCooperativeKernel((const void *)solve_coop_kernel, dim3(pl->team),
                                          dim3(pl->block), args, 0, pl->stream));
       int done = 0;
@@ -511,7 +556,7 @@ static int plan_run(xcs_plan *pl, float *ms) {
     }
   } else {
     int budget = P.maxiter + 4;
-    void *args[] = {(void *)&pl->d, (void *)&budget, (void *)&pl->red};
+    void *args[] = {(void *)&pl->d, (void *)&budget, (void *)&pl->red, (void *)&pl->bar};
     HIP_TRY(hipLaunchCooperativeKernel((const void *)solve_coop_kernel, dim3(pl->team),
                                        dim3(pl->block), args, 0, pl->stream));
   }
@@ -521,6 +566,9 @@ static int plan_run(xcs_plan *pl, float *ms) {
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   HIP_TRY(hipMemcpy(&pl->last, P.st, sizeof(XState), hipMemcpyDeviceToHost));
+  unsigned abort_flag = 0;
+  HIP_TRY(hipMemcpy(&abort_flag, pl->bar + 2, sizeof(unsigned), hipMemcpyDeviceToHost));
+  if (abort_flag) return set_err(XCS_EHIP, "team barrier timed out (results invalid)");
   if (ms) *ms = pl->last_ms;
   return XCS_OK;
 }
@@ -572,6 +620,333 @@ static void host_norms(const std::vector<float> &xt, const float *xo, int k, flo
     support[2] = tn;
     support[3] = n - k - tn;
   }
+}
+
+// ----------------------------------------------------------------- batches --
+struct xcs_batch {
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  int count = 0, c = 1, block = 256, nteams = 0, grid = 0, budget = 0;
+  std::vector<XProblem> hP;
+  XProblem *dP = nullptr, *teamP = nullptr;
+  XWork *dW = nullptr;
+  int *slots = nullptr, *queue = nullptr;
+  double *red = nullptr;
+  unsigned *bar = nullptr;
+  std::vector<void *> allocs;
+  std::vector<std::vector<float>> x_true;
+  std::vector<int> ks;
+  std::vector<XState> last;
+  std::vector<int> order;  // order[q] = caller index of queue slot q
+  unsigned long long *phase = nullptr;  // XCS_PHASE_TIMING builds only
+  float gen_ms = 0.f, last_ms = 0.f;
+  ~xcs_batch() {
+    for (void *q : allocs) (void)hipFree(q);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+  template <class T>
+  int alloc(T **q, size_t cnt) {
+    void *r = nullptr;
+    const size_t bytes = std::max<size_t>(cnt, 1) * sizeof(T);
+    hipError_t e = hipMalloc(&r, bytes);
+    if (e != hipSuccess)
+      return set_err(XCS_ENOMEM, "hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(e));
+    allocs.push_back(r);
+    e = hipMemsetAsync(r, 0, bytes, stream);
+    if (e != hipSuccess) return set_err(XCS_EHIP, "hipMemsetAsync: %s", hipGetErrorString(e));
+    *q = (T *)r;
+    return XCS_OK;
+  }
+};
+
+static int batch_occupancy(int block, int *per_cu, int *cus) {
+  int dev;
+  HIP_TRY(hipGetDevice(&dev));
+  HIP_TRY(hipDeviceGetAttribute(cus, hipDeviceAttributeMultiprocessorCount, dev));
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, (const void *)solve_batch_kernel,
+                                                       block, 0));
+  if (*per_cu <= 0) return set_err(XCS_EINVAL, "block size %d does not fit on a CU", block);
+  return XCS_OK;
+}
+
+static int batch_generate(xcs_batch *B, int alg, int count, const int *ens, const int *k,
+                          const int *m, const int *n, const int *p, const unsigned *inst,
+                          const xcs_options *o) {
+  if (alg < 0) return set_err(XCS_EINVAL, "algorithm must be one of NIHT, HTP, CSMPSP, CGIHT");
+  if (count <= 0) return set_err(XCS_EINVAL, "count must be > 0");
+  for (int i = 0; i < count; i++) {
+    XCS_TRY(validate(alg, ens[i], k[i], m[i], n[i]));
+    if (k[i] > n[i]) return set_err(XCS_EINVAL, "instance %d: k > n", i);
+    if (ens[i] == ENS_SMV && (!p || p[i] <= 0 || p[i] > m[i]))
+      return set_err(XCS_EINVAL, "instance %d: smv needs 0 < p <= m", i);
+  }
+  XCS_TRY(use_device(o->gpu_number));
+  HIP_TRY(hipGetDevice(&B->dev));
+  HIP_TRY(hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking));
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  HIP_TRY(hipEventRecord(e0, B->stream));
+  B->count = count;
+  B->c = o->team_size > 0 ? o->team_size : 1;
+  B->block = o->block_size > 0 ? o->block_size : 256;
+  if (B->block % 64 || B->block > 1024) return set_err(XCS_EINVAL, "block_size must be a multiple of 64 <= 1024");
+  int per_cu, cus;
+  XCS_TRY(batch_occupancy(B->block, &per_cu, &cus));
+  const int resident = per_cu * cus;
+  if (B->c > resident) return set_err(XCS_EINVAL, "team_size %d exceeds %d resident workgroups", B->c, resident);
+  int nteams = std::min(count, resident / B->c);
+  if (B->c > 1 && nteams >= 8) nteams -= nteams % 8;  // whole teams per XCD
+  B->nteams = nteams;
+  B->grid = nteams * B->c;
+  int n_max = 1, lda_max = 64, bins_max = 1000;
+  for (int i = 0; i < count; i++) {
+    n_max = std::max(n_max, n[i]);
+    lda_max = std::max(lda_max, round_up(m[i], 64));
+    bins_max = std::max(bins_max, o->num_bins > 0 ? o->num_bins : std::max(n[i] / 20, 1000));
+  }
+  bool any_dct = false;
+  for (int i = 0; i < count; i++) any_dct |= ens[i] == ENS_DCT;
+  // per-team workspace
+  std::vector<XWork> hw((size_t)nteams);
+  {
+    const size_t T = (size_t)nteams, nn = (size_t)n_max, ml = (size_t)lda_max;
+    float *f_n, *f_m, *part = nullptr;
+    int *i_n, *cnt, *wgc;
+    float2 *cw = nullptr;
+    XCS_TRY(B->alloc(&f_n, T * 4 * nn));
+    XCS_TRY(B->alloc(&i_n, T * 3 * nn));
+    XCS_TRY(B->alloc(&f_m, T * 3 * ml));
+    XCS_TRY(B->alloc(&cnt, T * ((size_t)bins_max + 64)));
+    XCS_TRY(B->alloc(&wgc, T * ((size_t)B->c + 64)));
+    XCS_TRY(B->alloc(&part, T * MAXCH * ml));
+    if (any_dct) XCS_TRY(B->alloc(&cw, T * 3 * nn));
+    for (size_t t = 0; t < T; t++) {
+      XWork &W = hw[t];
+      xset(W.thres, f_n + (t * 4 + 0) * nn);
+      xset(W.grad, f_n + (t * 4 + 1) * nn);
+      xset(W.gprev, f_n + (t * 4 + 2) * nn);
+      xset(W.gprev_thres, f_n + (t * 4 + 3) * nn);
+      xset(W.bins, i_n + (t * 3 + 0) * nn);
+      xset(W.bins2, i_n + (t * 3 + 1) * nn);
+      xset(W.supp, i_n + (t * 3 + 2) * nn);
+      xset(W.resid, f_m + (t * 3 + 0) * ml);
+      xset(W.resid_update, f_m + (t * 3 + 1) * ml);
+      xset(W.resid2, f_m + (t * 3 + 2) * ml);
+      xset(W.counters, cnt + t * ((size_t)bins_max + 64));
+      xset(W.wg_counts, wgc + t * ((size_t)B->c + 64));
+      xset(W.partials, part + t * MAXCH * ml);
+      xset(W.cwork, cw ? cw + t * 3 * nn : nullptr);
+    }
+    XCS_TRY(B->alloc(&B->dW, T));
+    HIP_TRY(hipMemcpyAsync(B->dW, hw.data(), sizeof(XWork) * T, hipMemcpyHostToDevice, B->stream));
+    XCS_TRY(B->alloc(&B->teamP, T));
+    XCS_TRY(B->alloc(&B->slots, T));
+    XCS_TRY(B->alloc(&B->queue, 1));
+    XCS_TRY(B->alloc(&B->red, T * 2 * RED_SLOTS * (size_t)B->c));
+    XCS_TRY(B->alloc(&B->bar, T * BAR_WORDS));
+  }
+  // instances
+  B->hP.resize((size_t)count);
+  B->x_true.resize((size_t)count);
+  B->ks.assign(k, k + count);
+  B->last.resize((size_t)count);
+  // helper problem for y = A x (team 0's workspace, one workgroup)
+  XProblem *d_tmp;
+  XCS_TRY(B->alloc(&d_tmp, 1));
+  double *tmp_red;
+  unsigned *tmp_bar;
+  XCS_TRY(B->alloc(&tmp_red, 2 * RED_SLOTS * 2));
+  XCS_TRY(B->alloc(&tmp_bar, BAR_WORDS));
+  std::vector<std::pair<int, const float2 *>> tw_cache;
+  int maxiter_default = 0;
+  for (int i = 0; i < count; i++) {
+    XProblem &P = B->hP[i];
+    memset(&P, 0, sizeof(P));
+    P.alg = alg;
+    P.ens = ens[i];
+    P.m = m[i];
+    P.n = n[i];
+    P.k = k[i];
+    P.p = ens[i] == ENS_SMV ? p[i] : 0;
+    P.lda = round_up(m[i], 64);
+    P.num_bins = o->num_bins > 0 ? o->num_bins : std::max(n[i] / 20, 1000);
+    P.maxiter = o->maxiter > 0 ? o->maxiter : ((alg == ALG_HTP || alg == ALG_CSMPSP) ? 300 : 5000);
+    maxiter_default = std::max(maxiter_default, P.maxiter);
+    P.restart = o->restart;
+    P.tol = o->tol_reference_default ? (float)(10 ^ (-4)) : o->tol;
+    P.log2n = ens[i] == ENS_DCT ? ilog2_exact(n[i]) : 0;
+    const uint32_t seed = o->seed, key1 = inst ? inst[i] : (uint32_t)i;
+    float *y, *x;
+    XState *st;
+    XCS_TRY(B->alloc(&y, (size_t)P.lda));
+    XCS_TRY(B->alloc(&x, (size_t)P.n));
+    XCS_TRY(B->alloc(&st, 1));
+    xset(P.y, y);
+    xset(P.x, x);
+    xset(P.st, st);
+    if (P.ens == ENS_GEN) {
+      float *A;
+      XCS_TRY(B->alloc(&A, (size_t)P.lda * P.n));
+      const size_t total = (size_t)P.lda * P.n;
+      const int blocks = (int)std::min<size_t>(65536, (total + 255) / 256);
+      hipLaunchKernelGGL(gen_matrix_kernel, dim3(blocks), dim3(256), 0, B->stream, A, P.m, P.n,
+                         P.lda, seed, key1);
+      HIP_TRY(hipGetLastError());
+      xset(P.A, A);
+    } else if (P.ens == ENS_SMV) {
+      std::vector<int> rows, cp((size_t)P.n + 1), row_ptr((size_t)P.m + 1, 0), cols;
+      std::vector<float> vals, rvals;
+      XCS_TRY(host_smv(P.m, P.n, P.p, o->matrix_ensemble, seed, key1, rows, vals));
+      const int nz = P.n * P.p;
+      for (int j = 0; j <= P.n; j++) cp[j] = j * P.p;
+      cols.resize((size_t)nz);
+      rvals.resize((size_t)nz);
+      for (int e = 0; e < nz; e++) row_ptr[rows[e] + 1]++;
+      for (int r = 0; r < P.m; r++) row_ptr[r + 1] += row_ptr[r];
+      std::vector<int> fill(row_ptr.begin(), row_ptr.end() - 1);
+      for (int j = 0; j < P.n; j++)
+        for (int e = cp[j]; e < cp[j + 1]; e++) {
+          const int pos = fill[rows[e]]++;
+          cols[pos] = j;
+          rvals[pos] = vals[e];
+        }
+      int *d_cp, *d_r, *d_rp, *d_c;
+      float *d_v, *d_rv;
+      XCS_TRY(B->alloc(&d_cp, (size_t)P.n + 1));
+      XCS_TRY(B->alloc(&d_r, (size_t)nz));
+      XCS_TRY(B->alloc(&d_v, (size_t)nz));
+      XCS_TRY(B->alloc(&d_rp, (size_t)P.m + 1));
+      XCS_TRY(B->alloc(&d_c, (size_t)nz));
+      XCS_TRY(B->alloc(&d_rv, (size_t)nz));
+      HIP_TRY(hipMemcpyAsync(d_cp, cp.data(), sizeof(int) * (P.n + 1), hipMemcpyHostToDevice, B->stream));
+      HIP_TRY(hipMemcpyAsync(d_r, rows.data(), sizeof(int) * nz, hipMemcpyHostToDevice, B->stream));
+      HIP_TRY(hipMemcpyAsync(d_v, vals.data(), sizeof(float) * nz, hipMemcpyHostToDevice, B->stream));
+      HIP_TRY(hipMemcpyAsync(d_rp, row_ptr.data(), sizeof(int) * (P.m + 1), hipMemcpyHostToDevice, B->stream));
+      HIP_TRY(hipMemcpyAsync(d_c, cols.data(), sizeof(int) * nz, hipMemcpyHostToDevice, B->stream));
+      HIP_TRY(hipMemcpyAsync(d_rv, rvals.data(), sizeof(float) * nz, hipMemcpyHostToDevice, B->stream));
+      HIP_TRY(hipStreamSynchronize(B->stream));  // host vectors go out of scope
+      xset(P.col_ptr, d_cp);
+      xset(P.rows, d_r);
+      xset(P.vals, d_v);
+      xset(P.row_ptr, d_rp);
+      xset(P.cols, d_c);
+      xset(P.rvals, d_rv);
+      P.nz = nz;
+    } else {
+      std::vector<int> rows;
+      host_subset(P.n, P.m, seed, key1, STREAM_DCT_ROWS, rows);
+      int *d_rows;
+      XCS_TRY(B->alloc(&d_rows, (size_t)P.m));
+      HIP_TRY(hipMemcpyAsync(d_rows, rows.data(), sizeof(int) * P.m, hipMemcpyHostToDevice, B->stream));
+      HIP_TRY(hipStreamSynchronize(B->stream));
+      xset(P.dct_rows, d_rows);
+      const float2 *tw = nullptr;
+      for (auto &c : tw_cache)
+        if (c.first == P.n) tw = c.second;
+      if (!tw) {
+        float2 *t2;
+        XCS_TRY(B->alloc(&t2, 2 * (size_t)P.n));
+        hipLaunchKernelGGL(dct_tables_kernel, dim3(std::min(4096, (P.n + 255) / 256)), dim3(256), 0,
+                           B->stream, t2, P.n);
+        HIP_TRY(hipGetLastError());
+        tw_cache.push_back({P.n, t2});
+        tw = t2;
+      }
+      xset(P.twiddle, tw);
+    }
+    // planted vector and y = A x with team 0's workspace
+    std::vector<int> supp;
+    host_sparse_vector(P.n, P.k, o->vec_distribution, seed, key1, B->x_true[i], supp);
+    XProblem Q = P;
+    attach_work(Q, hw[0]);
+    float *d_xt = Q.thres;  // team-0 scratch holds x_true during the product
+    HIP_TRY(hipMemcpyAsync(d_xt, B->x_true[i].data(), sizeof(float) * P.n, hipMemcpyHostToDevice, B->stream));
+    if (!supp.empty())
+      HIP_TRY(hipMemcpyAsync(Q.supp, supp.data(), sizeof(int) * supp.size(), hipMemcpyHostToDevice, B->stream));
+    HIP_TRY(hipMemcpyAsync(d_tmp, &Q, sizeof(XProblem), hipMemcpyHostToDevice, B->stream));
+    int transpose = 0, cnt = (int)supp.size();
+    const float *in = d_xt;
+    float *out = y;
+    void *args[] = {(void *)&d_tmp, (void *)&transpose, (void *)&in, (void *)&out, (void *)&cnt,
+                    (void *)&tmp_red, (void *)&tmp_bar};
+    HIP_TRY(hipLaunchCooperativeKernel((const void *)apply_coop_kernel, dim3(1), dim3(256), args, 0,
+                                       B->stream));
+    HIP_TRY(hipStreamSynchronize(B->stream));  // host x_true/supp reused next instance
+  }
+  B->budget = maxiter_default + 4;
+#ifdef XCS_PHASE_TIMING
+  XCS_TRY(B->alloc(&B->phase, 32));
+  for (XProblem &P : B->hP) xset(P.phase_ticks, B->phase);
+#endif
+  // largest instances first (work-queue load balance)
+  std::vector<int> order((size_t)count);
+  for (int i = 0; i < count; i++) order[i] = i;
+  auto cost = [&](int i) {
+    const XProblem &P = B->hP[i];
+    return P.ens == ENS_GEN ? (double)P.lda * P.n : P.ens == ENS_SMV ? 2.0 * P.nz : 6.0 * P.n;
+  };
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost(a) > cost(b); });
+  std::vector<XProblem> sorted((size_t)count);
+  for (int q = 0; q < count; q++) sorted[q] = B->hP[order[q]];
+  // keep the caller's indexing for results: remember the order
+  B->hP = sorted;
+  {
+    std::vector<std::vector<float>> xt2((size_t)count);
+    std::vector<int> k2((size_t)count);
+    for (int q = 0; q < count; q++) {
+      xt2[q].swap(B->x_true[order[q]]);
+      k2[q] = B->ks[order[q]];
+    }
+    B->x_true.swap(xt2);
+    B->ks.swap(k2);
+  }
+  B->order = order;
+  XCS_TRY(B->alloc(&B->dP, (size_t)count));
+  HIP_TRY(hipMemcpyAsync(B->dP, B->hP.data(), sizeof(XProblem) * count, hipMemcpyHostToDevice, B->stream));
+  HIP_TRY(hipEventRecord(e1, B->stream));
+  HIP_TRY(hipEventSynchronize(e1));
+  HIP_TRY(hipEventElapsedTime(&B->gen_ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return XCS_OK;
+}
+
+static int batch_run(xcs_batch *B, float *ms) {
+  for (const XProblem &P : B->hP) {
+    HIP_TRY(hipMemsetAsync(P.x, 0, sizeof(float) * P.n, B->stream));
+    HIP_TRY(hipMemsetAsync(P.st, 0, sizeof(XState), B->stream));
+  }
+  HIP_TRY(hipMemsetAsync(B->queue, 0, sizeof(int), B->stream));
+  HIP_TRY(hipMemsetAsync(B->bar, 0, sizeof(unsigned) * BAR_WORDS * B->nteams, B->stream));
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  HIP_TRY(hipEventRecord(e0, B->stream));
+  void *args[] = {(void *)&B->dP, (void *)&B->count, (void *)&B->queue, (void *)&B->teamP,
+                  (void *)&B->dW, (void *)&B->slots, (void *)&B->red, (void *)&B->bar,
+                  (void *)&B->c, (void *)&B->budget};
+  if (B->c > 1) {
+    HIP_TRY(hipLaunchCooperativeKernel((const void *)solve_batch_kernel, dim3(B->grid),
+                                       dim3(B->block), args, 0, B->stream));
+  } else {
+    HIP_TRY(hipLaunchKernel((const void *)solve_batch_kernel, dim3(B->grid), dim3(B->block), args,
+                            0, B->stream));
+  }
+  HIP_TRY(hipEventRecord(e1, B->stream));
+  HIP_TRY(hipEventSynchronize(e1));
+  HIP_TRY(hipEventElapsedTime(&B->last_ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  for (int q = 0; q < B->count; q++)
+    HIP_TRY(hipMemcpyAsync(&B->last[q], B->hP[q].st, sizeof(XState), hipMemcpyDeviceToHost, B->stream));
+  std::vector<unsigned> bars((size_t)BAR_WORDS * B->nteams);
+  HIP_TRY(hipMemcpyAsync(bars.data(), B->bar, sizeof(unsigned) * bars.size(), hipMemcpyDeviceToHost, B->stream));
+  HIP_TRY(hipStreamSynchronize(B->stream));
+  for (int t = 0; t < B->nteams; t++)
+    if (bars[(size_t)t * BAR_WORDS + 2]) return set_err(XCS_EHIP, "team barrier timed out (results invalid)");
+  if (ms) *ms = B->last_ms;
+  return XCS_OK;
 }
 
 // ------------------------------------------------------------------ C ABI --
@@ -648,6 +1023,55 @@ int xcs_plan_trace(xcs_plan *pl, int *k_bin, float *resid_norm, int len) {
   return XCS_OK;
 }
 
+int xcs_plan_apply_timed(xcs_plan *pl, int transpose, int reps, float *ms_per, double *bytes) {
+  if (!pl || reps <= 0 || !ms_per) return set_err(XCS_EINVAL, "bad arguments");
+  HIP_TRY(hipSetDevice(pl->dev));
+  XProblem &P = pl->h;
+  // input: the planted product y (A^T y) or the planted vector's support (A x_S)
+  const float *in = transpose ? (const float *)P.y : (const float *)P.thres;
+  float *out = transpose ? (float *)P.grad : (float *)P.resid2;
+  int cnt = 0;
+  if (!transpose) {
+    std::vector<int> supp;
+    for (int j = 0; j < P.n; j++)
+      if (!pl->x_true.empty() && pl->x_true[j] != 0.f) supp.push_back(j);
+    cnt = (int)supp.size();
+    HIP_TRY(hipMemcpyAsync((void *)P.thres, pl->x_true.data(), sizeof(float) * P.n, hipMemcpyHostToDevice, pl->stream));
+    if (cnt)
+      HIP_TRY(hipMemcpyAsync((void *)P.supp, supp.data(), sizeof(int) * cnt, hipMemcpyHostToDevice, pl->stream));
+  }
+  XCS_TRY(plan_upload_desc(pl));
+  XCS_TRY(launch_apply(pl, transpose, in, out, cnt));  // warm-up
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  HIP_TRY(hipEventRecord(e0, pl->stream));
+  for (int r = 0; r < reps; r++) XCS_TRY(launch_apply(pl, transpose, in, out, cnt));
+  HIP_TRY(hipEventRecord(e1, pl->stream));
+  HIP_TRY(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *ms_per = ms / reps;
+  if (bytes) {
+    const double m = P.m, n = P.n;
+    if (P.ens == ENS_GEN)
+      *bytes = transpose ? 4.0 * m * n + 4.0 * m + 4.0 * n : 4.0 * m * cnt + 8.0 * cnt + 4.0 * m;
+    else if (P.ens == ENS_SMV)
+      *bytes = 8.0 * P.nz + 4.0 * m + 4.0 * n;
+    else
+      *bytes = 4.0 * n + 8.0 * m;
+  }
+  return XCS_OK;
+}
+
+int xcs_plan_bytes(xcs_plan *pl, double *bytes) {
+  if (!pl || !bytes) return set_err(XCS_EINVAL, "NULL argument");
+  *bytes = pl->last.mat_bytes;
+  return XCS_OK;
+}
+
 void xcs_plan_destroy(xcs_plan *pl) { delete pl; }
 
 int xcs_cs_generate(const char *alg, const char *ens, int k, int m, int n, int p,
@@ -680,7 +1104,7 @@ static int given_solve(xcs_plan *pl, const float *y, float *x_out, int *iteratio
   float *d_y;
   XCS_TRY(pl->alloc(&d_y, (size_t)P.lda));
   HIP_TRY(hipMemcpyAsync(d_y, y, sizeof(float) * P.m, hipMemcpyHostToDevice, pl->stream));
-  P.y = d_y;
+  xset(P.y, d_y);
   XCS_TRY(plan_run(pl, nullptr));
   if (x_out) HIP_TRY(hipMemcpy(x_out, P.x, sizeof(float) * P.n, hipMemcpyDeviceToHost));
   if (iterations) *iterations = pl->last.iter;
@@ -712,7 +1136,7 @@ int xcs_cs_gen(const char *alg, const float *y, const float *A, int m, int n, in
   XCS_TRY(pl.alloc(&dA, (size_t)pl.h.lda * n));
   HIP_TRY(hipMemcpy2DAsync(dA, sizeof(float) * pl.h.lda, A, sizeof(float) * m, sizeof(float) * m, n,
                            hipMemcpyHostToDevice, pl.stream));
-  pl.h.A = dA;
+  xset(pl.h.A, dA);
   return given_solve(&pl, y, x_out, iterations, conv_rate);
 }
 
@@ -775,7 +1199,7 @@ int xcs_cs_dct(const char *alg, const float *y, const int *rows, int k, int m, i
   int *d_rows;
   XCS_TRY(pl.alloc(&d_rows, (size_t)m));
   HIP_TRY(hipMemcpyAsync(d_rows, r0.data(), sizeof(int) * m, hipMemcpyHostToDevice, pl.stream));
-  pl.h.dct_rows = d_rows;
+  xset(pl.h.dct_rows, d_rows);
   return given_solve(&pl, y, x_out, iterations, conv_rate);
 }
 
@@ -803,7 +1227,7 @@ int xcs_find_support_set(const float *v, int n, int k, int num_bins, float slope
   HIP_TRY(hipMemcpyAsync(pl.h.st, &S, sizeof(S), hipMemcpyHostToDevice, pl.stream));
   XCS_TRY(plan_upload_desc(&pl));
   void *args[] = {(void *)&pl.d, (void *)&dv, (void *)&pl.h.st, (void *)&slope,
-                  (void *)&intercept, (void *)&pl.red};
+                  (void *)&intercept, (void *)&pl.red, (void *)&pl.bar};
   HIP_TRY(hipLaunchCooperativeKernel((const void *)find_support_coop_kernel, dim3(pl.team),
                                      dim3(pl.block), args, 0, pl.stream));
   HIP_TRY(hipStreamSynchronize(pl.stream));
@@ -840,7 +1264,7 @@ int xcs_apply(const char *ens, int transpose, int m, int n, const float *A, cons
     XCS_TRY(pl.alloc(&dA, (size_t)P.lda * n));
     HIP_TRY(hipMemcpy2DAsync(dA, sizeof(float) * P.lda, A, sizeof(float) * m, sizeof(float) * m, n,
                              hipMemcpyHostToDevice, pl.stream));
-    P.A = dA;
+    xset(P.A, dA);
   } else if (e == ENS_SMV) {
     std::vector<int> cp(col_ptr, col_ptr + n + 1), rr(rows, rows + nz);
     std::vector<float> vv(vals, vals + nz);
@@ -854,7 +1278,7 @@ int xcs_apply(const char *ens, int transpose, int m, int n, const float *A, cons
     int *dr;
     XCS_TRY(pl.alloc(&dr, (size_t)m));
     HIP_TRY(hipMemcpyAsync(dr, dct_rows, sizeof(int) * m, hipMemcpyHostToDevice, pl.stream));
-    P.dct_rows = dr;
+    xset(P.dct_rows, dr);
   }
   const int lin = transpose ? m : n, lout = transpose ? n : m;
   float *din, *dout;
@@ -890,6 +1314,87 @@ int xcs_generate_gen(int m, int n, unsigned int seed, unsigned int instance, flo
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(pl.stream));
   HIP_TRY(hipMemcpy(A, dA, sizeof(float) * total, hipMemcpyDeviceToHost));
+  return XCS_OK;
+}
+
+
+int xcs_batch_create(const char *alg, int count, const int *ens, const int *k, const int *m,
+                     const int *n, const int *p, const unsigned int *instance,
+                     const xcs_options *opts, xcs_batch **out) {
+  if (!out || !opts || !ens || !k || !m || !n) return set_err(XCS_EINVAL, "NULL argument");
+  *out = nullptr;
+  xcs_batch *B = new xcs_batch();
+  int r = batch_generate(B, parse_alg(alg), count, ens, k, m, n, p, instance, opts);
+  if (r != XCS_OK) {
+    delete B;
+    return r;
+  }
+  *out = B;
+  return XCS_OK;
+}
+
+int xcs_batch_solve(xcs_batch *B, float *ms) {
+  if (!B) return set_err(XCS_EINVAL, "NULL batch");
+  HIP_TRY(hipSetDevice(B->dev));
+  return batch_run(B, ms);
+}
+
+int xcs_batch_results(xcs_batch *B, float *norms, int *iterations, int *stop_reason,
+                      float *conv_rate) {
+  if (!B) return set_err(XCS_EINVAL, "NULL batch");
+  HIP_TRY(hipSetDevice(B->dev));
+  std::vector<float> xo;
+  for (int q = 0; q < B->count; q++) {
+    const int i = B->order[q];
+    const XProblem &P = B->hP[q];
+    if (norms) {
+      xo.resize((size_t)P.n);
+      HIP_TRY(hipMemcpy(xo.data(), P.x, sizeof(float) * P.n, hipMemcpyDeviceToHost));
+      host_norms(B->x_true[q], xo.data(), B->ks[q], norms + 3 * (size_t)i, nullptr);
+    }
+    if (iterations) iterations[i] = B->last[q].iter;
+    if (stop_reason) stop_reason[i] = B->last[q].stop_reason;
+    if (conv_rate) conv_rate[i] = conv_rate_results(B->last[q]);
+  }
+  return XCS_OK;
+}
+
+int xcs_batch_x(xcs_batch *B, int idx, float *x_out, float *x_true) {
+  if (!B || idx < 0 || idx >= B->count) return set_err(XCS_EINVAL, "bad batch index");
+  HIP_TRY(hipSetDevice(B->dev));
+  int q = 0;
+  while (B->order[q] != idx) q++;
+  const XProblem &P = B->hP[q];
+  if (x_out) HIP_TRY(hipMemcpy(x_out, P.x, sizeof(float) * P.n, hipMemcpyDeviceToHost));
+  if (x_true) memcpy(x_true, B->x_true[q].data(), sizeof(float) * P.n);
+  return XCS_OK;
+}
+
+int xcs_batch_info(xcs_batch *B, int *teams, int *team_size, int *block, float *gen_ms) {
+  if (!B) return set_err(XCS_EINVAL, "NULL batch");
+  if (teams) *teams = B->nteams;
+  if (team_size) *team_size = B->c;
+  if (block) *block = B->block;
+  if (gen_ms) *gen_ms = B->gen_ms;
+  return XCS_OK;
+}
+
+int xcs_batch_bytes(xcs_batch *B, double *bytes) {
+  if (!B || !bytes) return set_err(XCS_EINVAL, "NULL argument");
+  double s = 0.0;
+  for (const XState &S : B->last) s += S.mat_bytes;
+  *bytes = s;
+  return XCS_OK;
+}
+
+void xcs_batch_destroy(xcs_batch *B) { delete B; }
+
+// Phase tick counters of the instrumented build (zeros otherwise).
+int xcs_batch_phase_ticks(xcs_batch *B, unsigned long long *out16) {
+  if (!B || !out16) return set_err(XCS_EINVAL, "NULL argument");
+  memset(out16, 0, 32 * sizeof(unsigned long long));
+  if (B->phase)
+    HIP_TRY(hipMemcpy(out16, B->phase, 32 * sizeof(unsigned long long), hipMemcpyDeviceToHost));
   return XCS_OK;
 }
 

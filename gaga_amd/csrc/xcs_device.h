@@ -17,10 +17,77 @@
 
 namespace xcs {
 
+// Device data pointers are typed as global (address space 1) in the device
+// pass.  Descriptors are read from LDS; without the qualifier the compiler
+// cannot prove where their pointers point and issues flat loads, which share
+// the LDS wait counter and serialise LDS reads behind HBM loads.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define XG(T) T __attribute__((address_space(1))) *
+#else
+#define XG(T) T *
+#endif
+
 constexpr int WAVE = 64;
 constexpr int MAX_WAVES = 16;  // blockDim.x <= 1024
 
 // ---------------------------------------------------------------- team ----
+constexpr int BAR_GROUP = 16;     // workgroups per first-level barrier counter
+constexpr int BAR_WORDS = 128;    // per-team barrier words: gen, top, abort, groups...
+constexpr long BAR_SPIN_LIMIT = 1L << 26;
+
+// Hierarchical counter barrier over `size` workgroups.  bar[0] generation,
+// bar[1] top-level count, bar[2] abort flag, bar[4 + g] group counts.
+// Every wave fences (agent scope) before arriving so its global stores are
+// visible to the other workgroups (s_barrier alone does not wait for them);
+// after the release every wave fences again (acquire).  The spin is bounded:
+// on timeout the abort flag is raised and every later barrier returns at once,
+// so a fault in the protocol ends the kernel instead of hanging the GPU.
+__device__ __forceinline__ void team_barrier(unsigned *bar, int rank, int size) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (__hip_atomic_load(&bar[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
+      const unsigned gen = __hip_atomic_load(&bar[0], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+      const int ngroups = (size + BAR_GROUP - 1) / BAR_GROUP;
+      bool last;
+      if (ngroups > 1) {
+        const int g = rank / BAR_GROUP;
+        const int gsz = min(BAR_GROUP, size - g * BAR_GROUP);
+        unsigned prev = __hip_atomic_fetch_add(&bar[4 + g], 1u, __ATOMIC_ACQ_REL,
+                                               __HIP_MEMORY_SCOPE_AGENT);
+        last = (prev == (unsigned)(gsz - 1));
+        if (last) {
+          __hip_atomic_store(&bar[4 + g], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          prev = __hip_atomic_fetch_add(&bar[1], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+          last = (prev == (unsigned)(ngroups - 1));
+        }
+      } else {
+        const unsigned prev = __hip_atomic_fetch_add(&bar[1], 1u, __ATOMIC_ACQ_REL,
+                                                     __HIP_MEMORY_SCOPE_AGENT);
+        last = (prev == (unsigned)(size - 1));
+      }
+      if (last) {
+        __hip_atomic_store(&bar[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&bar[0], gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      } else {
+        long spins = 0;
+        while (__hip_atomic_load(&bar[0], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+          __builtin_amdgcn_s_sleep(1);
+          if (++spins > BAR_SPIN_LIMIT) {
+            __hip_atomic_store(&bar[2], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+          }
+          if ((spins & 1023) == 0 &&
+              __hip_atomic_load(&bar[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u)
+            break;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
 struct Team {
   int rank;  // workgroup rank inside the team
   int size;  // number of workgroups in the team
@@ -28,13 +95,14 @@ struct Team {
   // RED_SLOTS * size doubles (bank b at red + b * RED_SLOTS * size)
   double *red;
   int bank;
+  unsigned *bar;  // BAR_WORDS barrier words of this team (size > 1)
   __device__ int gtid() const { return rank * (int)blockDim.x + (int)threadIdx.x; }
   __device__ int gsize() const { return size * (int)blockDim.x; }
   __device__ int gwave() const { return rank * ((int)blockDim.x / WAVE) + (int)threadIdx.x / WAVE; }
   __device__ int nwaves() const { return size * ((int)blockDim.x / WAVE); }
   __device__ void sync() const {
     if (size > 1)
-      cooperative_groups::this_grid().sync();
+      team_barrier(bar, rank, size);
     else
       __syncthreads();
   }
@@ -166,7 +234,7 @@ __device__ __forceinline__ int linear_bin(float v, float slope, float intercept)
 // The sequential k-bin walk of FindSupportSet (functions.cu:414-420) done as a
 // workgroup-parallel scan over counters[0, max_bin).  Every workgroup that
 // calls it computes the same (k_bin, sum).  counters must be complete.
-__device__ void kbin_select(const int *counters, int max_bin, int k, int *k_bin_out,
+__device__ void kbin_select(XG(const int) counters, int max_bin, int k, int *k_bin_out,
                             int *sum_out) {
   __shared__ int res[2];
   if (k <= 0) {  // the reference loop body never runs: k_bin = -1, sum = 0

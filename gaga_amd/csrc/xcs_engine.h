@@ -4,6 +4,12 @@
 
 namespace xcs {
 
+// Host-side assignment of a plain pointer to a descriptor field.
+template <class D, class S>
+__host__ __device__ inline void xset(D &dst, S src) {
+  dst = (D)src;
+}
+
 enum Alg { ALG_NIHT = 0, ALG_HTP = 1, ALG_CSMPSP = 2, ALG_CGIHT = 3 };
 enum Ens { ENS_GEN = 0, ENS_SMV = 1, ENS_DCT = 2 };
 
@@ -15,34 +21,81 @@ struct XState {
   int beta_to_zero, supp_change, fail, cycle, pad0;
   float alpha_bin, min_val, max_change, max_value, slope, mu, err, err_start, diff, pad1;
   float prev[16], evo[16];
+  double mat_bytes;  // algorithmic bytes moved by the operator applications of this solve
 };
 
 // Everything a team needs to solve one instance.  Device pointers only.
 struct XProblem {
   int alg, ens, m, n, k, lda, num_bins, maxiter, restart, p, nz, log2n;
   float tol;
-  const float *A;  // gen: column-major, leading dimension lda (rows m..lda-1 are 0)
-  const int *col_ptr, *rows;  // smv CSC (n+1, nz)
-  const float *vals;
-  const int *row_ptr, *cols;  // smv CSR (m+1, nz)
-  const float *rvals;
-  const int *dct_rows;        // dct: m sampled rows of the DCT-II
-  const float *y;             // length lda, zero padded
-  float *x;                   // n
-  float *thres, *grad, *gprev, *gprev_thres;  // n
-  int *bins, *bins2;                          // n
-  float *resid, *resid_update, *resid2;       // lda each, zero padded
-  int *counters;                              // >= num_bins
-  int *supp;                                  // n: ascending support list
-  int *wg_counts;                             // team size
-  float *partials;                            // MAXCH * lda (gen gather partials)
-  float2 *cwork;                              // dct: 3 * n complex scratch
-  const float2 *twiddle;                      // dct: n roots of unity, then n quarter-wave
-  XState *st;
-  float *trace_err;                           // optional, maxiter + 2
-  int *trace_kbin;                            // optional, maxiter + 2
+  XG(const float) A;  // gen: column-major, leading dimension lda (rows m..lda-1 are 0)
+  XG(const int) col_ptr;  // smv CSC (n+1, nz)
+  XG(const int) rows;
+  XG(const float) vals;
+  XG(const int) row_ptr;  // smv CSR (m+1, nz)
+  XG(const int) cols;
+  XG(const float) rvals;
+  XG(const int) dct_rows;     // dct: m sampled rows of the DCT-II
+  XG(const float) y;          // length lda, zero padded
+  XG(float) x;                // n
+  XG(float) thres;            // n-vectors
+  XG(float) grad;
+  XG(float) gprev;
+  XG(float) gprev_thres;
+  XG(int) bins;               // n
+  XG(int) bins2;
+  XG(float) resid;            // lda each, zero padded
+  XG(float) resid_update;
+  XG(float) resid2;
+  XG(int) counters;           // >= num_bins
+  XG(int) supp;               // n: ascending support list
+  XG(int) wg_counts;          // team size
+  XG(float) partials;         // MAXCH * lda (gen gather partials)
+  XG(float2) cwork;           // dct: 3 * n complex scratch
+  XG(const float2) twiddle;   // dct: n roots of unity, then n quarter-wave
+  XG(XState) st;
+  XG(float) trace_err;        // optional, maxiter + 2
+  XG(int) trace_kbin;         // optional, maxiter + 2
+  XG(unsigned long long) phase_ticks;  // optional (XCS_PHASE_TIMING builds): 32 counters
 };
 
 constexpr int MAXCH = 64;  // gen gather: max column chunks (partials rows)
+
+// Scratch owned by a team (not by an instance): the batched engine reuses it
+// for every instance the team solves.
+struct XWork {
+  XG(float) thres;
+  XG(float) grad;
+  XG(float) gprev;
+  XG(float) gprev_thres;
+  XG(int) bins;
+  XG(int) bins2;
+  XG(float) resid;
+  XG(float) resid_update;
+  XG(float) resid2;
+  XG(int) counters;
+  XG(int) supp;
+  XG(int) wg_counts;
+  XG(float) partials;
+  XG(float2) cwork;
+};
+
+// Point an instance's scratch fields at a team's workspace.
+__host__ __device__ inline void attach_work(XProblem &P, const XWork &W) {
+  P.thres = W.thres;
+  P.grad = W.grad;
+  P.gprev = W.gprev;
+  P.gprev_thres = W.gprev_thres;
+  P.bins = W.bins;
+  P.bins2 = W.bins2;
+  P.resid = W.resid;
+  P.resid_update = W.resid_update;
+  P.resid2 = W.resid2;
+  P.counters = W.counters;
+  P.supp = W.supp;
+  P.wg_counts = W.wg_counts;
+  P.partials = W.partials;
+  P.cwork = W.cwork;
+}
 
 }  // namespace xcs

@@ -18,6 +18,7 @@ namespace xcs {
 constexpr int RLDS = 8192;   // gen: residual staged in LDS when lda <= RLDS
 constexpr int HLDS = 4096;   // binning: histogram in LDS when max_bin <= HLDS
 constexpr int FLDS = 4096;   // FFT: complex elements held in LDS
+constexpr int SLDS = 4096;   // gen gather: support entries staged in LDS per pass
 
 union Lds {
   struct {
@@ -25,13 +26,17 @@ union Lds {
     int hist[HLDS];
   } s;
   float2 f[FLDS];
+  struct {
+    int j[SLDS];
+    float v[SLDS];
+  } g;
 };
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
   return make_float2(__fsub_rn(__fmul_rn(a.x, b.x), __fmul_rn(a.y, b.y)),
                      __fadd_rn(__fmul_rn(a.x, b.y), __fmul_rn(a.y, b.x)));
 }
-__device__ __forceinline__ float2 twid(const float2 *tw, int n, long long e, int sign) {
+__device__ __forceinline__ float2 twid(XG(const float2) tw, int n, long long e, int sign) {
   // tw[t] = exp(-2 pi i t / n); sign +1 (inverse) conjugates.
   const int t = (int)(e & (long long)(n - 1));
   float2 w = tw[t];
@@ -41,7 +46,7 @@ __device__ __forceinline__ float2 twid(const float2 *tw, int n, long long e, int
 
 // cnt sequences of length len (power of two) stored contiguously in LDS at
 // f[s*len + bitrev(j)]; in-place iterative radix-2 DIT.  len*cnt <= FLDS.
-__device__ void lds_fft(float2 *f, int len, int cnt, const float2 *tw, int n, int sign) {
+__device__ void lds_fft(float2 *f, int len, int cnt, XG(const float2) tw, int n, int sign) {
   const int half = len >> 1;
   for (int span = 1; span < len; span <<= 1) {
     const int stride = n / (span << 1);  // w_{2 span}^pos = w_n^(pos * stride)
@@ -67,8 +72,8 @@ __device__ __forceinline__ int bitrev(int v, int bits) {
 // out = FFT(in) of length n = 2^log2n (sign -1 forward, +1 unnormalised
 // inverse).  in and out distinct, both length n.  Caller makes `in` visible;
 // ends with a team barrier (out visible).  `mid` is scratch of length n.
-__device__ void team_fft(Team &t, Lds &L, const float2 *in, float2 *mid, float2 *out,
-                         const float2 *tw, int n, int log2n, int sign) {
+__device__ __noinline__ void team_fft(Team &t, Lds &L, XG(const float2) in, XG(float2) mid, XG(float2) out,
+                         XG(const float2) tw, int n, int log2n, int sign) {
   if (n <= FLDS) {
     if (t.rank == 0) {
       for (int j = threadIdx.x; j < n; j += blockDim.x) L.f[bitrev(j, log2n)] = in[j];
@@ -131,10 +136,12 @@ __device__ __forceinline__ float dct_w(const XProblem &P, int q) {
 }
 
 // y = (DCT-II_ortho v)[rows]  (A_dct, dct_gpu.cu:328-338).  Strided out.
-__device__ void dct_A(Team &t, Lds &L, const XProblem &P, const float *v, float *out) {
+__device__ __noinline__ void dct_A(Team &t, Lds &L, const XProblem &P, XG(const float) v, XG(float) out) {
   const int n = P.n;
-  float2 *z = P.cwork, *mid = P.cwork + n, *Z = P.cwork + 2 * (size_t)n;
-  const float2 *qw = P.twiddle + n;  // (cos, sin)(pi q / 2n)
+  XG(float2) z = P.cwork;
+  XG(float2) mid = P.cwork + n;
+  XG(float2) Z = P.cwork + 2 * (size_t)n;
+  XG(const float2) qw = P.twiddle + n;  // (cos, sin)(pi q / 2n)
   for (int q = t.gtid(); q < n / 2; q += t.gsize()) {
     z[q] = make_float2(v[2 * q], 0.f);
     z[n - 1 - q] = make_float2(v[2 * q + 1], 0.f);
@@ -151,10 +158,12 @@ __device__ void dct_A(Team &t, Lds &L, const XProblem &P, const float *v, float 
 // g = DCT-III_ortho(zero-fill(r at rows))  (AT_dct, dct_gpu.cu:341-353).
 // epi(j, g_j) by the owner of j (strided over pairs).
 template <class Epi>
-__device__ void dct_AT(Team &t, Lds &L, const XProblem &P, const float *r, Epi &&epi) {
+__device__ void dct_AT(Team &t, Lds &L, const XProblem &P, XG(const float) r, Epi &&epi) {
   const int n = P.n;
-  float2 *z = P.cwork, *mid = P.cwork + n, *Y = P.cwork + 2 * (size_t)n;
-  const float2 *qw = P.twiddle + n;
+  XG(float2) z = P.cwork;
+  XG(float2) mid = P.cwork + n;
+  XG(float2) Y = P.cwork + 2 * (size_t)n;
+  XG(const float2) qw = P.twiddle + n;
   for (int q = t.gtid(); q < n; q += t.gsize()) z[q] = make_float2(0.f, 0.f);
   t.sync();
   for (int i = t.gtid(); i < P.m; i += t.gsize()) {

@@ -24,99 +24,190 @@
 
 namespace xcs {
 
+// Optional phase timing (compile with -DXCS_PHASE_TIMING): workgroup 0 of a
+// team accumulates wall-clock ticks per phase into P.phase_ticks[id].
+#ifdef XCS_PHASE_TIMING
+#define PHASE_BEGIN() unsigned long long ph_t0_ = wall_clock64()
+#define PHASE(P, t, id)                                                      \
+  do {                                                                     \
+    const unsigned long long ph_t1_ = wall_clock64();                      \
+    if ((P).phase_ticks && (t).rank == 0 && threadIdx.x == 0)              \
+      atomicAdd(&(P).phase_ticks[id], ph_t1_ - ph_t0_);                    \
+    ph_t0_ = ph_t1_;                                                       \
+  } while (0)
+#else
+#define PHASE_BEGIN() \
+  do {              \
+  } while (0)
+#define PHASE(P, t, id) \
+  do {                  \
+  } while (0)
+#endif
+
 // ------------------------------------------------------------ gen ----------
 // out[i] = sum_{s < cnt} A[i, supp[s]] * v[supp[s]]   (gen_matvec_sparse,
 // GAGA/Matrices/gen_gpu.cu:79-132, as a deterministic two-phase gather).
-// out is written with strided ownership.  Reads v: caller makes it visible.
-__device__ void gen_A(Team &t, const XProblem &P, const float *v, float *out, int cnt) {
+// The support indices and values are staged in LDS so every column load is
+// independent; work items are (256-row tile, chunk of cs support columns);
+// chunk partials are summed in a fixed order.  out is written with strided
+// ownership.  Reads v: caller makes it visible.
+__device__ __noinline__ void gen_A(Team &t, Lds &L, const XProblem &P, XG(const float) v,
+                                   XG(float) out,
+                      int cnt) {
   const int lane = threadIdx.x & (WAVE - 1);
-  const int nq = P.lda >> 2;
+  const int lda = P.lda, nq = lda >> 2;
+  XG(const float) A = P.A;
+  XG(const int) supp = P.supp;
+  XG(float) part = P.partials;
   const int row_tiles = (nq + WAVE - 1) / WAVE;
   int cs = (cnt + MAXCH - 1) / MAXCH;
   if (cs < 8) cs = 8;
   const int nch = (cnt + cs - 1) / cs;
-  const int items = row_tiles * nch;
-  for (int it = t.gwave(); it < items; it += t.nwaves()) {
-    const int rt = it % row_tiles, ch = it / row_tiles;
-    const int q = rt * WAVE + lane;
-    if (q < nq) {
-      float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-      const int s1 = min(cnt, ch * cs + cs);
-      for (int s = ch * cs; s < s1; s++) {
-        const int j = P.supp[s];
-        const float xv = v[j];
-        const float4 a = reinterpret_cast<const float4 *>(P.A + (size_t)j * P.lda)[q];
-        acc.x = fmaf(a.x, xv, acc.x);
-        acc.y = fmaf(a.y, xv, acc.y);
-        acc.z = fmaf(a.z, xv, acc.z);
-        acc.w = fmaf(a.w, xv, acc.w);
-      }
-      reinterpret_cast<float4 *>(P.partials + (size_t)ch * P.lda)[q] = acc;
+  const int cpp = SLDS / cs;  // whole chunks per LDS pass
+#ifdef XCS_PHASE_TIMING
+  if (P.phase_ticks && t.rank == 0 && threadIdx.x == 0) {
+    atomicMax(&P.phase_ticks[22], (unsigned long long)cnt);
+    atomicAdd(&P.phase_ticks[23], (unsigned long long)cnt);
+    atomicAdd(&P.phase_ticks[24], 1ull);
+  }
+#endif
+  PHASE_BEGIN();
+  for (int ch0 = 0; ch0 < nch; ch0 += cpp) {
+    const int s_base = ch0 * cs;
+    const int s_cnt = min(cnt - s_base, cpp * cs);
+    const int ch_cnt = min(nch - ch0, cpp);
+    for (int i = threadIdx.x; i < s_cnt; i += blockDim.x) {
+      const int j = supp[s_base + i];
+      L.g.j[i] = j;
+      L.g.v[i] = v[j];
     }
+    __syncthreads();
+    PHASE(P, t, 18);
+    const int items = row_tiles * ch_cnt;
+    for (int it = t.gwave(); it < items; it += t.nwaves()) {
+      const int rt = it % row_tiles, ch = it / row_tiles;
+      const int q = rt * WAVE + lane;
+      if (q < nq) {
+        float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+        const int s0 = ch * cs, s1 = min(s_cnt, s0 + cs);
+        int s = s0;
+        for (; s + 4 <= s1; s += 4) {
+          float4 a[4];
+          float xv[4];
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            a[u] = reinterpret_cast<XG(const float4)>(A + (size_t)L.g.j[s + u] * lda)[q];
+            xv[u] = L.g.v[s + u];
+          }
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            acc.x = fmaf(a[u].x, xv[u], acc.x);
+            acc.y = fmaf(a[u].y, xv[u], acc.y);
+            acc.z = fmaf(a[u].z, xv[u], acc.z);
+            acc.w = fmaf(a[u].w, xv[u], acc.w);
+          }
+        }
+        for (; s < s1; s++) {
+          const float4 a = reinterpret_cast<XG(const float4)>(A + (size_t)L.g.j[s] * lda)[q];
+          const float xv = L.g.v[s];
+          acc.x = fmaf(a.x, xv, acc.x);
+          acc.y = fmaf(a.y, xv, acc.y);
+          acc.z = fmaf(a.z, xv, acc.z);
+          acc.w = fmaf(a.w, xv, acc.w);
+        }
+        reinterpret_cast<XG(float4)>(part + (size_t)(ch0 + ch) * lda)[q] = acc;
+      }
+    }
+    PHASE(P, t, 19);
+    __syncthreads();
+    PHASE(P, t, 20);
   }
   t.sync();
   for (int i = t.gtid(); i < P.m; i += t.gsize()) {
     float s = 0.f;
-    for (int c = 0; c < nch; c++) s += P.partials[(size_t)c * P.lda + i];
+    for (int c = 0; c < nch; c++) s += part[(size_t)c * lda + i];
     out[i] = s;
   }
+  PHASE(P, t, 21);
 }
 
 // g = A^T r, dense (gen_matvec on the transpose, gen_gpu.cu:33-74, 161-171),
-// one wave per column, reading the single column-major copy of A.
+// reading the single column-major copy of A.  Each wave streams two columns
+// at a time (four float4 per lane per column in flight, non-temporal: A is
+// read once per product) against the residual staged in LDS.
 // epi(j, g_j) is called once per column by lane 0 of the owning wave.
+typedef float xcs_v4f __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 load_nt(XG(const float4) p) {
+  const xcs_v4f v = __builtin_nontemporal_load(reinterpret_cast<XG(const xcs_v4f)>(p));
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ float dot4(float4 a, float4 b, float acc) {
+  acc = fmaf(a.x, b.x, acc);
+  acc = fmaf(a.y, b.y, acc);
+  acc = fmaf(a.z, b.z, acc);
+  return fmaf(a.w, b.w, acc);
+}
 template <class Epi>
-__device__ void gen_AT(Team &t, Lds &L, const XProblem &P, const float *r, Epi &&epi) {
+__device__ void gen_AT(Team &t, Lds &L, const XProblem &P, XG(const float) r, Epi &&epi) {
   const int lane = threadIdx.x & (WAVE - 1);
-  const int nq = P.lda >> 2;
-  const float4 *r4 = reinterpret_cast<const float4 *>(r);
-  if (P.lda <= RLDS) {
+  const int lda = P.lda, n = P.n, nq = lda >> 2;
+  XG(const float) A = P.A;
+  XG(const float4) r4 = reinterpret_cast<XG(const float4)>(r);
+  const bool lds = lda <= RLDS;
+  if (lds) {
     for (int q = threadIdx.x; q < nq; q += blockDim.x) L.s.r[q] = r4[q];
     __syncthreads();
-    for (int j = t.gwave(); j < P.n; j += t.nwaves()) {
-      const float4 *col = reinterpret_cast<const float4 *>(P.A + (size_t)j * P.lda);
-      float acc = 0.f;
-      int q = lane;
-      for (; q + 3 * WAVE < nq; q += 4 * WAVE) {
-        const float4 a0 = col[q], a1 = col[q + WAVE], a2 = col[q + 2 * WAVE], a3 = col[q + 3 * WAVE];
-        const float4 b0 = L.s.r[q], b1 = L.s.r[q + WAVE], b2 = L.s.r[q + 2 * WAVE], b3 = L.s.r[q + 3 * WAVE];
-        acc = fmaf(a0.x, b0.x, acc); acc = fmaf(a0.y, b0.y, acc);
-        acc = fmaf(a0.z, b0.z, acc); acc = fmaf(a0.w, b0.w, acc);
-        acc = fmaf(a1.x, b1.x, acc); acc = fmaf(a1.y, b1.y, acc);
-        acc = fmaf(a1.z, b1.z, acc); acc = fmaf(a1.w, b1.w, acc);
-        acc = fmaf(a2.x, b2.x, acc); acc = fmaf(a2.y, b2.y, acc);
-        acc = fmaf(a2.z, b2.z, acc); acc = fmaf(a2.w, b2.w, acc);
-        acc = fmaf(a3.x, b3.x, acc); acc = fmaf(a3.y, b3.y, acc);
-        acc = fmaf(a3.z, b3.z, acc); acc = fmaf(a3.w, b3.w, acc);
+  }
+  const int gw = t.gwave(), GW = t.nwaves();
+  constexpr int U = 4;
+  for (int j0 = 2 * gw; j0 < n; j0 += 2 * GW) {
+    const bool two = j0 + 1 < n;
+    XG(const float4) c0 = reinterpret_cast<XG(const float4)>(A + (size_t)j0 * lda);
+    XG(const float4) c1 = reinterpret_cast<XG(const float4)>(A + (size_t)(two ? j0 + 1 : j0) * lda);
+    float s00 = 0.f, s01 = 0.f, s10 = 0.f, s11 = 0.f;
+    for (int q = lane; q < nq; q += U * WAVE) {
+      float4 a0[U], a1[U], b[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int qq = q + u * WAVE;
+        const bool ok = qq < nq;
+        a0[u] = ok ? load_nt(c0 + qq) : make_float4(0.f, 0.f, 0.f, 0.f);
+        a1[u] = ok ? load_nt(c1 + qq) : make_float4(0.f, 0.f, 0.f, 0.f);
       }
-      for (; q < nq; q += WAVE) {
-        const float4 a = col[q], b = L.s.r[q];
-        acc = fmaf(a.x, b.x, acc); acc = fmaf(a.y, b.y, acc);
-        acc = fmaf(a.z, b.z, acc); acc = fmaf(a.w, b.w, acc);
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int qq = q + u * WAVE;
+        float4 rv = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (qq < nq) {
+          if (lds)
+            rv = L.s.r[qq];
+          else
+            rv = r4[qq];
+        }
+        b[u] = rv;
       }
-      acc = wave_sum(acc);
-      if (lane == 0) epi(j, acc);
+#pragma unroll
+      for (int u = 0; u < U; u += 2) {
+        s00 = dot4(a0[u], b[u], s00);
+        s01 = dot4(a0[u + 1], b[u + 1], s01);
+        s10 = dot4(a1[u], b[u], s10);
+        s11 = dot4(a1[u + 1], b[u + 1], s11);
+      }
     }
-    __syncthreads();
-  } else {
-    for (int j = t.gwave(); j < P.n; j += t.nwaves()) {
-      const float4 *col = reinterpret_cast<const float4 *>(P.A + (size_t)j * P.lda);
-      float acc = 0.f;
-      for (int q = lane; q < nq; q += WAVE) {
-        const float4 a = col[q], b = r4[q];
-        acc = fmaf(a.x, b.x, acc); acc = fmaf(a.y, b.y, acc);
-        acc = fmaf(a.z, b.z, acc); acc = fmaf(a.w, b.w, acc);
-      }
-      acc = wave_sum(acc);
-      if (lane == 0) epi(j, acc);
+    const float g0 = wave_sum(s00 + s01);
+    const float g1 = wave_sum(s10 + s11);
+    if (lane == 0) {
+      epi(j0, g0);
+      if (two) epi(j0 + 1, g1);
     }
   }
+  if (lds) __syncthreads();
 }
 
 // ------------------------------------------------------------ smv ----------
 // y = A v over the CSR copy (sparse_matvec, GAGA/Matrices/smv_gpu.cu:29-37,
 // as a deterministic row gather instead of atomics).  Strided ownership.
-__device__ void smv_A(Team &t, const XProblem &P, const float *v, float *out) {
+__device__ __noinline__ void smv_A(Team &t, const XProblem &P, XG(const float) v, XG(float) out) {
   for (int i = t.gtid(); i < P.m; i += t.gsize()) {
     float s = 0.f;
     const int e1 = P.row_ptr[i + 1];
@@ -126,7 +217,7 @@ __device__ void smv_A(Team &t, const XProblem &P, const float *v, float *out) {
 }
 // g = A^T r over CSC (sparse_matvecT, smv_gpu.cu:40-47): column gather.
 template <class Epi>
-__device__ void smv_AT(Team &t, const XProblem &P, const float *r, Epi &&epi) {
+__device__ void smv_AT(Team &t, const XProblem &P, XG(const float) r, Epi &&epi) {
   for (int j = t.gtid(); j < P.n; j += t.gsize()) {
     float s = 0.f;
     const int e1 = P.col_ptr[j + 1];
@@ -136,18 +227,31 @@ __device__ void smv_AT(Team &t, const XProblem &P, const float *r, Epi &&epi) {
 }
 
 // --------------------------------------------------------- dispatch --------
+// Algorithmic bytes of one operator application (what the roofline is quoted
+// against): gen A_S x_S reads |S| columns (4m|S| + 8|S|) and writes y (4m);
+// gen A^T r reads A once (4mn) plus r and g; smv reads values + row indices
+// (8 nz) plus both vectors; dct reads x and writes the m samples (4n + 8m).
+__device__ __forceinline__ double op_bytes(const XProblem &P, bool transpose, int cnt) {
+  const double m = P.m, n = P.n;
+  if (P.ens == ENS_GEN)
+    return transpose ? 4.0 * m * n + 4.0 * m + 4.0 * n : 4.0 * m * cnt + 8.0 * cnt + 4.0 * m;
+  if (P.ens == ENS_SMV) return 8.0 * P.nz + 4.0 * m + 4.0 * n;
+  return 4.0 * n + 8.0 * m;
+}
+
 // out = A v_S.  v must be zero off the support for smv/dct (it always is:
 // callers pass thresholded vectors).  Strided ownership of out.
-__device__ void op_A(Team &t, Lds &L, const XProblem &P, const float *v, float *out, int cnt) {
+__device__ void op_A(Team &t, Lds &L, const XProblem &P, XG(const float) v, XG(float) out,
+                     int cnt) {
   if (P.ens == ENS_GEN)
-    gen_A(t, P, v, out, cnt);
+    gen_A(t, L, P, v, out, cnt);
   else if (P.ens == ENS_SMV)
     smv_A(t, P, v, out);
   else
     dct_A(t, L, P, v, out);
 }
 template <class Epi>
-__device__ void op_AT(Team &t, Lds &L, const XProblem &P, const float *r, Epi &&epi) {
+__device__ void op_AT(Team &t, Lds &L, const XProblem &P, XG(const float) r, Epi &&epi) {
   if (P.ens == ENS_GEN)
     gen_AT(t, L, P, r, epi);
   else if (P.ens == ENS_SMV)
@@ -156,10 +260,23 @@ __device__ void op_AT(Team &t, Lds &L, const XProblem &P, const float *r, Epi &&
     dct_AT(t, L, P, r, epi);
 }
 
+// Operator applications of the solver, with algorithmic-byte accounting.
+__device__ __forceinline__ void opA(Team &t, Lds &L, const XProblem &P, XState &S,
+                                    XG(const float) v, XG(float) out) {
+  S.mat_bytes += op_bytes(P, false, S.supp_count);
+  op_A(t, L, P, v, out, S.supp_count);
+}
+template <class Epi>
+__device__ __forceinline__ void opAT(Team &t, Lds &L, const XProblem &P, XState &S,
+                                     XG(const float) r, Epi &&epi) {
+  S.mat_bytes += op_bytes(P, true, 0);
+  op_AT(t, L, P, r, epi);
+}
+
 // ------------------------------------------------------ support state ------
 // LinearBinning over v (kernels.cu:135-146); counters must be zero and
 // visible.  Ends with a team barrier (counters complete).
-__device__ void team_binning(Team &t, Lds &L, const XProblem &P, const float *v, int *bins,
+__device__ __noinline__ void team_binning(Team &t, Lds &L, const XProblem &P, XG(const float) v, XG(int) bins,
                              int max_bin, float slope, float intercept) {
   const bool lds = max_bin <= HLDS;
   if (lds) {
@@ -197,7 +314,8 @@ __device__ void zero_counters(Team &t, const XProblem &P) {
 
 // FindSupportSet's binning branch (functions.cu:404-428) on vector v whose
 // bins go to `bins`; scalars in S.  Caller zeroed counters and synced.
-__device__ void find_support_bin(Team &t, Lds &L, const XProblem &P, const float *v, int *bins,
+__device__ void find_support_bin(Team &t, Lds &L, const XProblem &P, XG(const float) v,
+                                 XG(int) bins,
                                  XState &S, float slope, float intercept, int *k_bin_out) {
   team_binning(t, L, P, v, bins, S.max_bin, slope, intercept);
   int kb, sum;
@@ -213,7 +331,7 @@ __device__ void find_support_bin(Team &t, Lds &L, const XProblem &P, const float
 
 // Ascending support list {i : bins[i] <= k_bin} into P.supp; returns its
 // length (uniform).  Requires bins visible; ends with supp visible.
-__device__ int build_support(Team &t, const XProblem &P, int k_bin) {
+__device__ __noinline__ int build_support(Team &t, const XProblem &P, int k_bin) {
   const int n = P.n;
   const int per = (n + t.size - 1) / t.size;
   const int lo = min(n, t.rank * per), hi = min(n, lo + per);
@@ -277,9 +395,9 @@ __device__ __forceinline__ float max16(const float *h) {
 }
 
 // resid = y - A x_S ; returns the FP64 partial of ||resid||^2 (strided).
-__device__ double residual(Team &t, Lds &L, const XProblem &P, const XState &S, float *resid,
-                           float *tmp) {
-  op_A(t, L, P, P.x, tmp, S.supp_count);
+__device__ double residual(Team &t, Lds &L, const XProblem &P, XState &S, XG(float) resid,
+                           XG(float) tmp) {
+  opA(t, L, P, S, P.x, tmp);
   double e2 = 0.0;
   for (int i = t.gtid(); i < P.m; i += t.gsize()) {
     const float r = __fsub_rn(P.y[i], tmp[i]);
@@ -307,7 +425,7 @@ __device__ float step_restricted_sd(Team &t, Lds &L, const XProblem &P, XState &
     push16(S.prev, (float)sqrt(v[0]));
   }
   float gmax = 0.f, th2 = 0.f;
-  op_AT(t, L, P, P.resid, [&](int j, float g) {
+  opAT(t, L, P, S, P.resid, [&](int j, float g) {
     P.grad[j] = g;
     const float th = (P.bins[j] <= kb) ? g : 0.f;
     P.thres[j] = th;
@@ -315,7 +433,7 @@ __device__ float step_restricted_sd(Team &t, Lds &L, const XProblem &P, XState &
     gmax = fmaxf(gmax, fabsf(g));
   });
   t.sync();
-  op_A(t, L, P, P.thres, P.resid_update, S.supp_count);
+  opA(t, L, P, S, P.thres, P.resid_update);
   float a2 = 0.f;
   for (int i = t.gtid(); i < P.m; i += t.gsize()) a2 = fmaf(P.resid_update[i], P.resid_update[i], a2);
   double v[3] = {a2, th2, gmax};
@@ -344,7 +462,7 @@ __device__ float step_restricted_sd(Team &t, Lds &L, const XProblem &P, XState &
 // Pushes the squared residual onto hist.
 __device__ void step_restricted_cg(Team &t, Lds &L, const XProblem &P, XState &S, float *hist) {
   const int kb = S.k_bin;
-  op_A(t, L, P, P.grad, P.resid, S.supp_count);
+  opA(t, L, P, S, P.grad, P.resid);
   float a2 = 0.f;
   for (int i = t.gtid(); i < P.m; i += t.gsize()) a2 = fmaf(P.resid[i], P.resid[i], a2);
   float alpha;
@@ -360,7 +478,7 @@ __device__ void step_restricted_cg(Team &t, Lds &L, const XProblem &P, XState &S
     alpha = 0.f;
   for (int i = t.gtid(); i < P.n; i += t.gsize()) P.x[i] = fmaf(alpha, P.grad[i], P.x[i]);
   float gp2 = 0.f;
-  op_AT(t, L, P, P.resid, [&](int j, float g) {
+  opAT(t, L, P, S, P.resid, [&](int j, float g) {
     const float th = (P.bins[j] <= kb) ? g : 0.f;
     const float gp = fmaf(-alpha, th, P.gprev[j]);
     P.gprev[j] = gp;
@@ -404,7 +522,7 @@ __device__ float cg_projection(Team &t, Lds &L, const XProblem &P, XState &S, bo
   double e2 = residual(t, L, P, S, P.resid, P.resid_update);
   t.sync();
   float g2 = 0.f;
-  op_AT(t, L, P, P.resid, [&](int j, float g) {
+  opAT(t, L, P, S, P.resid, [&](int j, float g) {
     const float th = (P.bins[j] <= kb) ? g : 0.f;
     P.grad[j] = th;
     P.gprev[j] = th;
@@ -432,20 +550,24 @@ __device__ float cg_projection(Team &t, Lds &L, const XProblem &P, XState &S, bo
 // RestrictedCGwithSupportEvolution_S_* (functions.cu:1004-1102).
 __device__ float step_cg_restricted(Team &t, Lds &L, const XProblem &P, XState &S, float *xmax) {
   const int kb = S.k_bin;
+  PHASE_BEGIN();
   double e2 = residual(t, L, P, S, P.resid, P.resid_update);
+  PHASE(P, t, 0);
   {
     double v[1] = {e2};
     const int op[1] = {0};
     reduce(t, v, op);
     push16(S.prev, (float)sqrt(v[0]));
   }
+  PHASE(P, t, 1);
   float th2 = 0.f;
-  op_AT(t, L, P, P.resid, [&](int j, float g) {
+  opAT(t, L, P, S, P.resid, [&](int j, float g) {
     P.grad[j] = g;
     const float th = (P.bins[j] <= kb) ? g : 0.f;
     P.thres[j] = th;
     th2 = fmaf(th, th, th2);
   });
+  PHASE(P, t, 2);
   float beta;
   {
     double v[1] = {th2};
@@ -472,7 +594,9 @@ __device__ float step_cg_restricted(Team &t, Lds &L, const XProblem &P, XState &
     pmax = fmaxf(pmax, fabsf(gp));
   }
   t.sync();
-  op_A(t, L, P, P.gprev_thres, P.resid_update, S.supp_count);
+  PHASE(P, t, 3);
+  opA(t, L, P, S, P.gprev_thres, P.resid_update);
+  PHASE(P, t, 4);
   float a2 = 0.f;
   for (int i = t.gtid(); i < P.m; i += t.gsize()) a2 = fmaf(P.resid_update[i], P.resid_update[i], a2);
   double v[2] = {a2, pmax};
@@ -490,14 +614,17 @@ __device__ float step_cg_restricted(Team &t, Lds &L, const XProblem &P, XState &
     xm = fmaxf(xm, fabsf(xn));
   }
   *xmax = xm;
+  PHASE(P, t, 5);
   return 2.0f * alpha * (float)v[1];
 }
 
 // UnrestrictedCG_S_* (functions.cu:1107-1220).
 __device__ float step_cg_unrestricted(Team &t, Lds &L, const XProblem &P, XState &S, float *xmax) {
   const int kb = S.k_bin;
+  PHASE_BEGIN();
   // resid_update = -(A x - y)
-  op_A(t, L, P, P.x, P.resid_update, S.supp_count);
+  opA(t, L, P, S, P.x, P.resid_update);
+  PHASE(P, t, 15);
   double e2 = 0.0;
   for (int i = t.gtid(); i < P.m; i += t.gsize()) {
     const float r = -__fsub_rn(P.resid_update[i], P.y[i]);
@@ -510,14 +637,18 @@ __device__ float step_cg_unrestricted(Team &t, Lds &L, const XProblem &P, XState
     reduce(t, v, op);
     push16(S.prev, (float)sqrt(v[0]));
   }
-  op_AT(t, L, P, P.resid_update, [&](int j, float g) {
+  PHASE(P, t, 6);
+  opAT(t, L, P, S, P.resid_update, [&](int j, float g) {
     P.grad[j] = g;
     P.thres[j] = (P.bins[j] <= kb) ? g : 0.f;
   });
   t.sync();
-  op_A(t, L, P, P.gprev_thres, P.resid, S.supp_count);
+  PHASE(P, t, 7);
+  opA(t, L, P, S, P.gprev_thres, P.resid);
   t.sync();
-  op_A(t, L, P, P.thres, P.resid_update, S.supp_count);
+  PHASE(P, t, 8);
+  opA(t, L, P, S, P.thres, P.resid_update);
+  PHASE(P, t, 9);
   float beta = 0.f;
   if (S.beta_to_zero != 1) {
     float num = 0.f, den = 0.f;
@@ -549,6 +680,7 @@ __device__ float step_cg_unrestricted(Team &t, Lds &L, const XProblem &P, XState
   double v[4] = {an, ad, pmax, mu2};
   const int op[4] = {0, 0, 1, 0};
   reduce(t, v, op);
+  PHASE(P, t, 10);
   float alpha = (float)v[0] / (float)v[1];
   float xm = 0.f;
   for (int i = t.gtid(); i < P.n; i += t.gsize()) {
@@ -653,7 +785,7 @@ __device__ void solve_instance(Team &t, Lds &L, const XProblem &P, int budget) {
     zero_counters(t, P);
     t.sync();
     float xm = 0.f;
-    op_AT(t, L, P, P.y, [&](int j, float g) {
+    opAT(t, L, P, S, P.y, [&](int j, float g) {
       P.x[j] = g;
       xm = fmaxf(xm, fabsf(g));
     });
@@ -692,7 +824,7 @@ __device__ void solve_instance(Team &t, Lds &L, const XProblem &P, int budget) {
       reduce(t, w, o1);
       S.err = (float)sqrt(w[0]);
       S.prev[15] = S.err;
-      op_AT(t, L, P, P.resid, [&](int j, float g) {
+      opAT(t, L, P, S, P.resid, [&](int j, float g) {
         P.gprev[j] = g;
         P.gprev_thres[j] = (P.bins[j] <= S.k_bin) ? g : 0.f;
       });
@@ -706,6 +838,9 @@ __device__ void solve_instance(Team &t, Lds &L, const XProblem &P, int budget) {
   }
 
   int reason = 0;
+#ifdef XCS_PHASE_TIMING
+  unsigned long long loop_t0 = wall_clock64();
+#endif
   while (budget > 0 && should_continue(S, P, &reason)) {
     budget--;
     if (P.alg == ALG_NIHT || P.alg == ALG_HTP) {
@@ -732,7 +867,7 @@ __device__ void solve_instance(Team &t, Lds &L, const XProblem &P, int budget) {
       // gradient support, joined with the current one (algorithms.cu:3524-3542)
       float gm = 0.f;
       zero_counters(t, P);
-      op_AT(t, L, P, P.resid, [&](int j, float g) {
+      opAT(t, L, P, S, P.resid, [&](int j, float g) {
         P.grad[j] = g;
         gm = fmaxf(gm, fabsf(g));
       });
@@ -767,14 +902,25 @@ __device__ void solve_instance(Team &t, Lds &L, const XProblem &P, int budget) {
       stop_update(S, 125);
     } else {  // CGIHT
       float xm;
-      if (P.restart || !S.supp_change)
+      if (P.restart || !S.supp_change) {
         S.max_change = step_cg_restricted(t, L, P, S, &xm);
-      else
+#ifdef XCS_PHASE_TIMING
+        if (P.phase_ticks && t.rank == 0 && threadIdx.x == 0) atomicAdd(&P.phase_ticks[16], 1ull);
+#endif
+      } else {
+        PHASE_BEGIN();
         S.max_change = step_cg_unrestricted(t, L, P, S, &xm);
+        PHASE(P, t, 11);
+#ifdef XCS_PHASE_TIMING
+        if (P.phase_ticks && t.rank == 0 && threadIdx.x == 0) atomicAdd(&P.phase_ticks[17], 1ull);
+#endif
+      }
       S.beta_to_zero = 0;
+      PHASE_BEGIN();
       if (S.min_val <= S.max_change) {
         const int kb_prev = S.k_bin;
         rebin_x(t, L, P, S, xm, true);
+        PHASE(P, t, 12);
         // checkSupportEvolution (kernels.cu:178-193), evaluated race-free
         float ch = 0.f;
         for (int i = t.gtid(); i < n; i += t.gsize())
@@ -794,11 +940,13 @@ __device__ void solve_instance(Team &t, Lds &L, const XProblem &P, int budget) {
         threshold_x(t, P, S.k_bin);
         t.sync();
         S.supp_count = build_support(t, P, S.k_bin);
+        PHASE(P, t, 12);
       } else {
         S.supp_change = 0;
         S.min_val = S.min_val - S.max_change;
         threshold_x(t, P, S.k_bin);
         t.sync();
+        PHASE(P, t, 12);
       }
       S.err = S.prev[15];
       stop_update(S, 749);
@@ -806,6 +954,12 @@ __device__ void solve_instance(Team &t, Lds &L, const XProblem &P, int budget) {
     S.iter++;
     trace_write(t, P, S.iter, S.k_bin, S.prev[15]);
   }
+#ifdef XCS_PHASE_TIMING
+  if (P.phase_ticks && t.rank == 0 && threadIdx.x == 0) {
+    atomicAdd(&P.phase_ticks[13], wall_clock64() - loop_t0);
+    atomicAdd(&P.phase_ticks[14], (unsigned long long)S.iter);
+  }
+#endif
   if (!should_continue(S, P, &reason)) {
     S.done = 1;
     S.stop_reason = reason;

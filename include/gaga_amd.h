@@ -94,7 +94,39 @@ int xcs_plan_results(xcs_plan *plan, float norms[3], int *iterations, int suppor
                      float *conv_rate, int *stop_reason, float *x_out, float *x_true);
 /* Per-iteration trace of the last solve: k_bin and residual norm, length iterations+1. */
 int xcs_plan_trace(xcs_plan *plan, int *k_bin, float *resid_norm, int len);
+/* Time `reps` back-to-back applications of A^T (transpose = 1, on the planted y) or A_S
+ * (transpose = 0, on the planted x) on the resident instance with the plan's team;
+ * *ms_per = device ms per application, *bytes = its algorithmic bytes. */
+int xcs_plan_apply_timed(xcs_plan *plan, int transpose, int reps, float *ms_per, double *bytes);
+/* Algorithmic bytes moved by the operator applications of the last solve
+ * (gen A_S x_S: 4m|S|+8|S|+4m; gen A^T r: 4mn+4m+4n; smv: 8nz+4m+4n; dct: 4n+8m). */
+int xcs_plan_bytes(xcs_plan *plan, double *bytes);
 void xcs_plan_destroy(xcs_plan *plan);
+
+/* ---- batched persistent form: many independent instances in one launch ---- */
+/* Ensemble codes for the per-instance arrays below. */
+enum { XCS_ENS_GEN = 0, XCS_ENS_SMV = 1, XCS_ENS_DCT = 2 };
+typedef struct xcs_batch xcs_batch;
+/* Generate `count` instances on the device (instance i: ensemble ens[i], sizes k[i], m[i],
+ * n[i], smv p[i]; generator key (opts->seed, instance ? instance[i] : i)).  One algorithm
+ * for all.  opts->team_size workgroups cooperate per instance (default 1); instances are
+ * taken largest first from a device queue (the phase-transition sweep driver,
+ * GAGA/CGIHTcode/shared/tests_delta_fixed.m:48-137, issues them one call at a time). */
+int xcs_batch_create(const char *alg, int count, const int *ens, const int *k, const int *m,
+                     const int *n, const int *p, const unsigned int *instance,
+                     const xcs_options *opts, xcs_batch **batch);
+/* Solve every instance from x = 0 in one persistent launch; *ms = device time. */
+int xcs_batch_solve(xcs_batch *batch, float *ms);
+/* Per-instance results, caller indexing: norms[3*i..3*i+2] relative l1/l2/linf. */
+int xcs_batch_results(xcs_batch *batch, float *norms, int *iterations, int *stop_reason,
+                      float *conv_rate);
+int xcs_batch_x(xcs_batch *batch, int index, float *x_out, float *x_true);
+int xcs_batch_info(xcs_batch *batch, int *teams, int *team_size, int *block_size, float *gen_ms);
+/* Algorithmic operator bytes of the last batch solve, summed over instances. */
+int xcs_batch_bytes(xcs_batch *batch, double *bytes);
+void xcs_batch_destroy(xcs_batch *batch);
+/* Phase tick counters (10 ns; 32 entries) of builds made with -DXCS_PHASE_TIMING; zeros otherwise. */
+int xcs_batch_phase_ticks(xcs_batch *batch, unsigned long long *ticks32);
 
 /* ---- kernel-level entry points (host buffers; used by parity tests) ---- */
 /* One FindSupportSet call (functions.cu:397-436) on the device:
