@@ -358,6 +358,9 @@ static int plan_alloc(xcs_plan *pl, int alg, int ens, int k, int m, int n, int p
     team = (int)std::min<double>(maxb, std::max(1.0, bytes / (256.0 * 1024.0)));
   }
   pl->team = std::min(team, maxb);
+  // CSMPSP runs on one workgroup per instance: its multi-workgroup team path
+  // faulted the GPU once and is switched off until the cause is shown.
+  if (alg == ALG_CSMPSP) pl->team = 1;
   const size_t nn = (size_t)n, ml = (size_t)P.lda;
   XCS_TRY(pl->alloc(&P.x, nn));
   XCS_TRY(pl->alloc(&P.thres, nn));
@@ -689,6 +692,7 @@ static int batch_generate(xcs_batch *B, int alg, int count, const int *ens, cons
   HIP_TRY(hipEventRecord(e0, B->stream));
   B->count = count;
   B->c = o->team_size > 0 ? o->team_size : 1;
+  if (alg == ALG_CSMPSP) B->c = 1;  // see plan_alloc: multi-workgroup CSMPSP is switched off
   B->block = o->block_size > 0 ? o->block_size : 256;
   if (B->block % 64 || B->block > 1024) return set_err(XCS_EINVAL, "block_size must be a multiple of 64 <= 1024");
   int per_cu, cus;

@@ -72,7 +72,7 @@ __device__ __forceinline__ int bitrev(int v, int bits) {
 // out = FFT(in) of length n = 2^log2n (sign -1 forward, +1 unnormalised
 // inverse).  in and out distinct, both length n.  Caller makes `in` visible;
 // ends with a team barrier (out visible).  `mid` is scratch of length n.
-__device__ __noinline__ void team_fft(Team &t, Lds &L, XG(const float2) in, XG(float2) mid, XG(float2) out,
+__device__ void team_fft(Team &t, Lds &L, XG(const float2) in, XG(float2) mid, XG(float2) out,
                          XG(const float2) tw, int n, int log2n, int sign) {
   if (n <= FLDS) {
     if (t.rank == 0) {
@@ -136,7 +136,7 @@ __device__ __forceinline__ float dct_w(const XProblem &P, int q) {
 }
 
 // y = (DCT-II_ortho v)[rows]  (A_dct, dct_gpu.cu:328-338).  Strided out.
-__device__ __noinline__ void dct_A(Team &t, Lds &L, const XProblem &P, XG(const float) v, XG(float) out) {
+__device__ void dct_A(Team &t, Lds &L, const XProblem &P, XG(const float) v, XG(float) out) {
   const int n = P.n;
   XG(float2) z = P.cwork;
   XG(float2) mid = P.cwork + n;

@@ -51,8 +51,7 @@ namespace xcs {
 // independent; work items are (256-row tile, chunk of cs support columns);
 // chunk partials are summed in a fixed order.  out is written with strided
 // ownership.  Reads v: caller makes it visible.
-__device__ __noinline__ void gen_A(Team &t, Lds &L, const XProblem &P, XG(const float) v,
-                                   XG(float) out,
+__device__ void gen_A(Team &t, Lds &L, const XProblem &P, XG(const float) v, XG(float) out,
                       int cnt) {
   const int lane = threadIdx.x & (WAVE - 1);
   const int lda = P.lda, nq = lda >> 2;
@@ -207,7 +206,7 @@ __device__ void gen_AT(Team &t, Lds &L, const XProblem &P, XG(const float) r, Ep
 // ------------------------------------------------------------ smv ----------
 // y = A v over the CSR copy (sparse_matvec, GAGA/Matrices/smv_gpu.cu:29-37,
 // as a deterministic row gather instead of atomics).  Strided ownership.
-__device__ __noinline__ void smv_A(Team &t, const XProblem &P, XG(const float) v, XG(float) out) {
+__device__ void smv_A(Team &t, const XProblem &P, XG(const float) v, XG(float) out) {
   for (int i = t.gtid(); i < P.m; i += t.gsize()) {
     float s = 0.f;
     const int e1 = P.row_ptr[i + 1];
@@ -276,7 +275,7 @@ __device__ __forceinline__ void opAT(Team &t, Lds &L, const XProblem &P, XState 
 // ------------------------------------------------------ support state ------
 // LinearBinning over v (kernels.cu:135-146); counters must be zero and
 // visible.  Ends with a team barrier (counters complete).
-__device__ __noinline__ void team_binning(Team &t, Lds &L, const XProblem &P, XG(const float) v, XG(int) bins,
+__device__ void team_binning(Team &t, Lds &L, const XProblem &P, XG(const float) v, XG(int) bins,
                              int max_bin, float slope, float intercept) {
   const bool lds = max_bin <= HLDS;
   if (lds) {
@@ -331,7 +330,7 @@ __device__ void find_support_bin(Team &t, Lds &L, const XProblem &P, XG(const fl
 
 // Ascending support list {i : bins[i] <= k_bin} into P.supp; returns its
 // length (uniform).  Requires bins visible; ends with supp visible.
-__device__ __noinline__ int build_support(Team &t, const XProblem &P, int k_bin) {
+__device__ int build_support(Team &t, const XProblem &P, int k_bin) {
   const int n = P.n;
   const int per = (n + t.size - 1) / t.size;
   const int lo = min(n, t.rank * per), hi = min(n, lo + per);

@@ -109,6 +109,18 @@ int orc32_solve(int alg, const orc_matrix *M, const float *y, int k, const orc_o
 int orc64_solve(int alg, const orc_matrix *M, const float *y, int k, const orc_opts *o,
                 double *x_out, orc_result *res, int *trace_kbin, double *trace_err);
 
+/* The HTP/CSMPSP inner CG projection (algorithms.cu:2607-2669) on a fixed
+ * support mask (mask[i] != 0: in S), from x = 0, at most 15 steps (the
+ * reference's maxiterCG), stopping at ||r||^2 <= tol^2.  Returns the steps run. */
+int orc64_cg_fixed(const orc_matrix *M, const float *y, const int *mask, float tol, double *x_out);
+
+/* One application of the stop-rule bookkeeping (algorithms.cu:1398-1421):
+ * shifts evo, sets evo[15] = prev[14] - prev[15], *diff = max(evo), clears
+ * *cycle when prev[15] repeats an earlier entry (iter > 4), sets *fail when
+ * iter > fail_after and (prev[15]/prev[0])^(1/15) > 0.999. */
+void orc_stop_update(const float prev[16], float evo[16], int iter, int fail_after, float *diff,
+                     int *cycle, int *fail);
+
 /* relative l1, l2, linf errors (results.cu:444-487) and TP,FP,TN,FN. */
 void orc_results(int n, int k, const float *x_true, const float *x_out, double norms[3],
                  int support[4]);

@@ -79,6 +79,9 @@ def _declare(L):
     L.orc32_solve.argtypes = [_i, _p, _p, _i, _p, _p, _p, _p, _p]
     L.orc64_solve.argtypes = [_i, _p, _p, _i, _p, _p, _p, _p, _p]
     L.orc_results.argtypes = [_i, _i, _p, _p, _p, _p]
+    L.orc64_cg_fixed.argtypes = [_p, _p, _p, _f, _p]
+    L.orc64_cg_fixed.restype = _i
+    L.orc_stop_update.argtypes = [_p, _p, _i, _i, _p, _p, _p]
 
 
 def _ptr(a):
@@ -239,3 +242,23 @@ def results(x_true, x_out, k):
     sup = np.zeros(4, dtype=np.int32)
     lib().orc_results(x_true.size, k, _ptr(x_true), _ptr(x_out), _ptr(norms), _ptr(sup))
     return norms, sup
+
+
+def cg_fixed(prob, y, mask, tol=1e-7):
+    """HTP/CSMPSP inner CG on a fixed support; returns (x, steps)."""
+    y = np.ascontiguousarray(y, dtype=np.float32)
+    mask = np.ascontiguousarray(mask, dtype=np.int32)
+    x = np.zeros(prob.n, dtype=np.float64)
+    steps = lib().orc64_cg_fixed(ctypes.byref(prob.M), _ptr(y), _ptr(mask), tol, _ptr(x))
+    return x, steps
+
+
+def stop_update(prev, evo, it, fail_after, cycle=1, fail=0):
+    """One stop-rule update; returns (evo, diff, cycle, fail)."""
+    prev = np.ascontiguousarray(prev, dtype=np.float32)
+    evo = np.array(evo, dtype=np.float32)
+    d = np.zeros(1, np.float32)
+    c = np.array([cycle], np.int32)
+    f = np.array([fail], np.int32)
+    lib().orc_stop_update(_ptr(prev), _ptr(evo), it, fail_after, _ptr(d), _ptr(c), _ptr(f))
+    return evo, float(d[0]), int(c[0]), int(f[0])

@@ -348,6 +348,79 @@ static void F(trace)(int *tk, REAL *te, int idx, int k_bin, REAL e) {
   if (te) te[idx] = e;
 }
 
+#if REAL_IS_DOUBLE
+int orc64_cg_fixed(const orc_matrix *M, const float *y_in, const int *mask, float tol,
+                   double *x_out) {
+  int m = M->m, n = M->n;
+  F(ws) W;
+  F(ws) *w = &W;
+  memset(w, 0, sizeof(W));
+  w->M = M;
+  w->m = m;
+  w->n = n;
+  w->x = x_out;
+  w->thres = (REAL *)calloc((size_t)n, sizeof(REAL));
+  w->grad = (REAL *)calloc((size_t)n, sizeof(REAL));
+  w->grad_prev = (REAL *)calloc((size_t)n, sizeof(REAL));
+  w->y = (REAL *)calloc((size_t)m, sizeof(REAL));
+  w->resid = (REAL *)calloc((size_t)m, sizeof(REAL));
+  w->resid_update = (REAL *)calloc((size_t)m, sizeof(REAL));
+  w->bins = (int *)calloc((size_t)n, sizeof(int));
+  for (int i = 0; i < m; i++) w->y[i] = y_in[i];
+  for (int i = 0; i < n; i++) {
+    x_out[i] = 0;
+    w->bins[i] = mask[i] ? 0 : 1;
+  }
+  REAL tolCG = (REAL)tol * (REAL)tol;
+  /* the loop of cg_projection(), returning the number of steps taken */
+  REAL hist[16], evo[16];
+  for (int j = 0; j < 16; j++) {
+    hist[j] = 0;
+    evo[j] = 1;
+  }
+  memcpy(w->resid, w->y, sizeof(REAL) * (size_t)m);
+  F(A)(M, w->x, w->resid_update);
+  F(axpy)(m, (REAL)-1, w->resid_update, w->resid);
+  F(AT)(M, w->resid, w->grad);
+  F(threshold)(n, w->grad, w->bins, 0);
+  memcpy(w->grad_prev, w->grad, sizeof(REAL) * (size_t)n);
+  REAL mu = F(dot)(n, w->grad_prev, w->grad_prev);
+  REAL errCG = F(dot)(m, w->resid, w->resid);
+  hist[15] = errCG;
+  int it = 0;
+  REAL diff = 1;
+  while ((errCG > tolCG) & (it < 15) & (diff > 10 * tolCG)) {
+    F(restricted_cg)(w, &mu, 0, hist);
+    errCG = hist[15];
+    it++;
+    for (int j = 0; j < 15; j++) evo[j] = evo[j + 1];
+    evo[15] = hist[14] - hist[15];
+    diff = F(max_list)(evo, 16);
+  }
+  free(w->thres);
+  free(w->grad);
+  free(w->grad_prev);
+  free(w->y);
+  free(w->resid);
+  free(w->resid_update);
+  free(w->bins);
+  return it;
+}
+#else
+void orc_stop_update(const float prev[16], float evo[16], int iter, int fail_after, float *diff,
+                     int *cycle, int *fail) {
+  F(stop) S;
+  memcpy(S.evo, evo, sizeof(S.evo));
+  S.cycle = *cycle;
+  S.fail = *fail;
+  F(stop_update)(&S, prev, iter, fail_after);
+  memcpy(evo, S.evo, sizeof(S.evo));
+  *diff = S.diff;
+  *cycle = S.cycle;
+  *fail = S.fail;
+}
+#endif
+
 int F(solve)(int alg, const orc_matrix *M, const float *y_in, int k, const orc_opts *o,
              REAL *x_out, orc_result *res, int *trace_kbin, REAL *trace_err) {
   int m = M->m, n = M->n;
