@@ -11,7 +11,7 @@ LIB_PATH = os.environ.get("XCS_LIB_PATH", os.path.join(_HERE, "libgaga_amd.so"))
 EXPORTS = (
     "xcs_options_default", "xcs_cs_generate", "xcs_cs_gen", "xcs_cs_smv", "xcs_cs_dct",
     "xcs_plan_create", "xcs_plan_solve", "xcs_plan_results", "xcs_plan_trace",
-    "xcs_plan_destroy", "xcs_plan_bytes", "xcs_plan_apply_timed", "xcs_batch_create", "xcs_batch_bytes", "xcs_batch_solve", "xcs_batch_results",
+    "xcs_plan_destroy", "xcs_plan_bytes", "xcs_plan_apply_timed", "xcs_batch_create", "xcs_batch_bytes", "xcs_batch_apply_timed", "xcs_batch_solve", "xcs_batch_results",
     "xcs_batch_x", "xcs_batch_info", "xcs_batch_destroy", "xcs_batch_phase_ticks",
     "xcs_find_support_set", "xcs_apply", "xcs_generate_gen",
     "xcs_last_error", "xcs_version",
@@ -65,6 +65,7 @@ def load():
         L.xcs_plan_bytes.argtypes = [_P, _P]
         L.xcs_plan_apply_timed.argtypes = [_P, _I, _I, _P, _P]
         L.xcs_batch_bytes.argtypes = [_P, _P]
+        L.xcs_batch_apply_timed.argtypes = [_P, _I, _P, _P]
         L.xcs_batch_create.argtypes = [_C, _I, _P, _P, _P, _P, _P, _P, _P, _P]
         L.xcs_batch_solve.argtypes = [_P, _P]
         L.xcs_batch_results.argtypes = [_P, _P, _P, _P, _P]
@@ -295,6 +296,12 @@ class Batch:
         b = ctypes.c_double(0)
         _ck(load().xcs_batch_bytes(self._h, ctypes.byref(b)))
         return b.value
+
+    def apply_timed(self, reps):
+        ms = ctypes.c_float(0)
+        b = ctypes.c_double(0)
+        _ck(load().xcs_batch_apply_timed(self._h, int(reps), ctypes.byref(ms), ctypes.byref(b)))
+        return ms.value, b.value
 
     def info(self):
         t, c, b, g = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_float()

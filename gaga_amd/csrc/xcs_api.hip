@@ -115,6 +115,27 @@ __global__ void __launch_bounds__(1024) apply_coop_kernel(const XProblem *P, int
   }
 }
 
+// A^T y (transpose = 1) or A_S x (0: support from the planted-vector list in
+// the team scratch is not available here, so only A^T is offered) for every
+// instance of a batch, one workgroup per instance: the streaming product of the
+// batched solver in isolation (bandwidth measurement).
+__global__ void __launch_bounds__(1024) apply_batch_kernel(const XProblem *Ps, int count,
+                                                           const XWork *works) {
+  __shared__ Lds L;
+  __shared__ XProblem sP;
+  Team t{0, 1, nullptr, 0, nullptr};
+  for (int idx = blockIdx.x; idx < count; idx += gridDim.x) {
+    if (threadIdx.x == 0) {
+      sP = Ps[idx];
+      attach_work(sP, works[blockIdx.x]);
+    }
+    __syncthreads();
+    XG(float) g = sP.grad;
+    op_AT(t, L, sP, sP.y, [&](int j, float v) { g[j] = v; });
+    __syncthreads();
+  }
+}
+
 // One FindSupportSet call with the team binning (functions.cu:397-436).
 __global__ void __launch_bounds__(1024) find_support_coop_kernel(const XProblem *P,
                                                                  const float *v, XState *S_io,
@@ -353,9 +374,13 @@ static int plan_alloc(xcs_plan *pl, int alg, int ens, int k, int m, int n, int p
   XCS_TRY(max_coop_blocks(pl->block, &maxb));
   int team = o->team_size;
   if (team <= 0) {
+    // Barrier cost grows with the team (about 0.1 us per arriving workgroup)
+    // while bandwidth grows with it: measured optima on MI355X were 32
+    // workgroups for gen 2048x8192, 64 for smv n=2^18, 128 for dct n=2^20.
     const double bytes = ens == ENS_GEN ? 4.0 * P.lda * (double)n
                          : ens == ENS_SMV ? 8.0 * nz : 24.0 * n;
-    team = (int)std::min<double>(maxb, std::max(1.0, bytes / (256.0 * 1024.0)));
+    const double per = ens == ENS_GEN ? 2.0e6 : ens == ENS_SMV ? 2.3e5 : 2.0e5;
+    team = (int)std::min<double>(128.0, std::max(1.0, std::round(bytes / per)));
   }
   pl->team = std::min(team, maxb);
   // CSMPSP runs on one workgroup per instance: its multi-workgroup team path
@@ -1380,6 +1405,40 @@ int xcs_batch_info(xcs_batch *B, int *teams, int *team_size, int *block, float *
   if (team_size) *team_size = B->c;
   if (block) *block = B->block;
   if (gen_ms) *gen_ms = B->gen_ms;
+  return XCS_OK;
+}
+
+int xcs_batch_apply_timed(xcs_batch *B, int reps, float *ms_per, double *bytes) {
+  if (!B || reps <= 0 || !ms_per) return set_err(XCS_EINVAL, "bad arguments");
+  HIP_TRY(hipSetDevice(B->dev));
+  int per_cu, cus;
+  XCS_TRY(batch_occupancy(B->block, &per_cu, &cus));
+  const int grid = std::min(B->count, B->nteams);  // works[] has one entry per team
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  hipLaunchKernelGGL(apply_batch_kernel, dim3(grid), dim3(B->block), 0, B->stream, B->dP, B->count, B->dW);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(e0, B->stream));
+  for (int r = 0; r < reps; r++)
+    hipLaunchKernelGGL(apply_batch_kernel, dim3(grid), dim3(B->block), 0, B->stream, B->dP, B->count, B->dW);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(e1, B->stream));
+  HIP_TRY(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *ms_per = ms / reps;
+  if (bytes) {
+    double s = 0.0;
+    for (const XProblem &P : B->hP) {
+      const double m = P.m, n = P.n;
+      s += P.ens == ENS_GEN ? 4.0 * m * n + 4.0 * m + 4.0 * n
+           : P.ens == ENS_SMV ? 8.0 * P.nz + 4.0 * m + 4.0 * n : 4.0 * n + 8.0 * m;
+    }
+    *bytes = s;
+  }
   return XCS_OK;
 }
 

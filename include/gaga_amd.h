@@ -122,6 +122,10 @@ int xcs_batch_results(xcs_batch *batch, float *norms, int *iterations, int *stop
                       float *conv_rate);
 int xcs_batch_x(xcs_batch *batch, int index, float *x_out, float *x_true);
 int xcs_batch_info(xcs_batch *batch, int *teams, int *team_size, int *block_size, float *gen_ms);
+/* Time `reps` launches that apply A^T y to every instance (one workgroup per instance,
+ * the batched solver's streaming product in isolation); *ms_per per launch, *bytes
+ * its algorithmic bytes summed over instances. */
+int xcs_batch_apply_timed(xcs_batch *batch, int reps, float *ms_per, double *bytes);
 /* Algorithmic operator bytes of the last batch solve, summed over instances. */
 int xcs_batch_bytes(xcs_batch *batch, double *bytes);
 void xcs_batch_destroy(xcs_batch *batch);
