@@ -11,7 +11,9 @@ LIB_PATH = os.environ.get("XCS_LIB_PATH", os.path.join(_HERE, "libgaga_amd.so"))
 EXPORTS = (
     "xcs_options_default", "xcs_cs_generate", "xcs_cs_gen", "xcs_cs_smv", "xcs_cs_dct",
     "xcs_plan_create", "xcs_plan_solve", "xcs_plan_results", "xcs_plan_trace",
-    "xcs_plan_destroy", "xcs_plan_bytes", "xcs_plan_apply_timed", "xcs_batch_create", "xcs_batch_bytes", "xcs_batch_apply_timed", "xcs_batch_solve", "xcs_batch_results",
+    "xcs_plan_destroy", "xcs_plan_bytes", "xcs_plan_apply_timed",
+    "xcs_batch_create", "xcs_batch_bytes", "xcs_batch_apply_timed", "xcs_batch_solve",
+    "xcs_batch_results", "xcs_batch_configure", "xcs_batch_tune",
     "xcs_batch_x", "xcs_batch_info", "xcs_batch_destroy", "xcs_batch_phase_ticks",
     "xcs_find_support_set", "xcs_apply", "xcs_generate_gen",
     "xcs_last_error", "xcs_version",
@@ -68,6 +70,8 @@ def load():
         L.xcs_batch_apply_timed.argtypes = [_P, _I, _P, _P]
         L.xcs_batch_create.argtypes = [_C, _I, _P, _P, _P, _P, _P, _P, _P, _P]
         L.xcs_batch_solve.argtypes = [_P, _P]
+        L.xcs_batch_configure.argtypes = [_P, _I, _I]
+        L.xcs_batch_tune.argtypes = [_P, _I, _C, _P, _P, _P]
         L.xcs_batch_results.argtypes = [_P, _P, _P, _P, _P]
         L.xcs_batch_x.argtypes = [_P, _I, _P, _P]
         L.xcs_batch_info.argtypes = [_P, _P, _P, _P, _P]
@@ -302,6 +306,18 @@ class Batch:
         b = ctypes.c_double(0)
         _ck(load().xcs_batch_apply_timed(self._h, int(reps), ctypes.byref(ms), ctypes.byref(b)))
         return ms.value, b.value
+
+    def configure(self, team_size, block_size):
+        _ck(load().xcs_batch_configure(self._h, int(team_size), int(block_size)))
+
+    def tune(self, probe_iters=32, cache=None):
+        """Pick the team geometry by timing candidates on this batch; returns
+        (team_size, block_size, probe_ms)."""
+        c, b, t = ctypes.c_int(), ctypes.c_int(), ctypes.c_float()
+        path = cache.encode() if cache else None
+        _ck(load().xcs_batch_tune(self._h, int(probe_iters), path, ctypes.byref(c),
+                                  ctypes.byref(b), ctypes.byref(t)))
+        return c.value, b.value, t.value
 
     def info(self):
         t, c, b, g = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_float()

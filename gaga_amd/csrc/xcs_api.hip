@@ -74,8 +74,7 @@ __device__ __forceinline__ void batch_team(int c, int *team_id, int *rank) {
 // Batched persistent form: teams of c workgroups take instances from a
 // queue and solve each to completion with team-scoped barriers only.
 __global__ void __launch_bounds__(1024) solve_batch_kernel(const XProblem *Ps, int count,
-                                                           int *queue, XProblem *teamP,
-                                                           const XWork *works, int *slots,
+                                                           int *queue, const XWork *works, int *slots,
                                                            double *red, unsigned *bar, int c,
                                                            int budget) {
   __shared__ Lds L;
@@ -134,6 +133,42 @@ __global__ void __launch_bounds__(1024) apply_batch_kernel(const XProblem *Ps, i
     op_AT(t, L, sP, sP.y, [&](int j, float v) { g[j] = v; });
     __syncthreads();
   }
+}
+
+// y = A x_true for every instance of a batch, one workgroup per instance
+// (the planted support is copied into the workgroup's support list first:
+// the gen product gathers the columns it names).
+__global__ void __launch_bounds__(1024) make_y_batch_kernel(const XProblem *Ps, int count,
+                                                            const XWork *works,
+                                                            const float *const *xt,
+                                                            const int *const *sp) {
+  __shared__ Lds L;
+  __shared__ XProblem sP;
+  Team t{0, 1, nullptr, 0, nullptr};
+  for (int idx = blockIdx.x; idx < count; idx += gridDim.x) {
+    if (threadIdx.x == 0) {
+      sP = Ps[idx];
+      attach_work(sP, works[blockIdx.x]);
+    }
+    __syncthreads();
+    const int *s = sp[idx];
+    for (int i = threadIdx.x; i < sP.k; i += blockDim.x) sP.supp[i] = s[i];
+    __syncthreads();
+    op_A(t, L, sP, (XG(const float))xt[idx], (XG(float))sP.y, sP.k);
+    __syncthreads();
+  }
+}
+
+// Start-of-solve reset of a batch: every x, every solver state, the work
+// queue and the team barriers (a kernel, so it can sit in a captured graph).
+__global__ void batch_reset_kernel(float *xs, size_t xs_len, int *st, size_t st_words, int *queue,
+                                   unsigned *bar, size_t bar_words) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  const size_t i0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (size_t i = i0; i < xs_len; i += stride) xs[i] = 0.f;
+  for (size_t i = i0; i < st_words; i += stride) st[i] = 0;
+  for (size_t i = i0; i < bar_words; i += stride) bar[i] = 0u;
+  if (i0 == 0) *queue = 0;
 }
 
 // One FindSupportSet call with the team binning (functions.cu:397-436).
@@ -651,39 +686,88 @@ static void host_norms(const std::vector<float> &xt, const float *xo, int k, flo
 }
 
 // ----------------------------------------------------------------- batches --
+// Instance data lives in two arenas: one for the dense gen matrices (written
+// in full by the generator, never cleared) and one for everything else
+// (cleared once).  All solution vectors are contiguous (one reset, one copy
+// back), and so are the solver states.
 struct xcs_batch {
   int dev = 0;
   hipStream_t stream = nullptr;
-  int count = 0, c = 1, block = 256, nteams = 0, grid = 0, budget = 0;
-  std::vector<XProblem> hP;
-  XProblem *dP = nullptr, *teamP = nullptr;
+  int alg = 0, count = 0, c = 1, block = 256, nteams = 0, grid = 0, budget = 0;
+  int n_max = 1, lda_max = 64, bins_max = 1000;
+  bool any_dct = false;
+  std::vector<XProblem> hP;  // queue order
+  XProblem *dP = nullptr;
+  float *xs = nullptr;       // every x, contiguous (queue order)
+  size_t xs_len = 0;
+  std::vector<size_t> xoff;
+  XState *sts = nullptr;
+  // team workspace (depends on c and block)
+  std::vector<void *> wallocs;
   XWork *dW = nullptr;
   int *slots = nullptr, *queue = nullptr;
   double *red = nullptr;
   unsigned *bar = nullptr;
+  hipGraphExec_t graph = nullptr;
   std::vector<void *> allocs;
   std::vector<std::vector<float>> x_true;
   std::vector<int> ks;
   std::vector<XState> last;
   std::vector<int> order;  // order[q] = caller index of queue slot q
+  std::vector<int> slot_of;  // slot_of[i] = queue slot of caller index i
   unsigned long long *phase = nullptr;  // XCS_PHASE_TIMING builds only
   float gen_ms = 0.f, last_ms = 0.f;
+  void drop_graph() {
+    if (graph) (void)hipGraphExecDestroy(graph);
+    graph = nullptr;
+  }
+  void drop_work() {
+    for (void *q : wallocs) (void)hipFree(q);
+    wallocs.clear();
+  }
   ~xcs_batch() {
+    drop_graph();
+    drop_work();
     for (void *q : allocs) (void)hipFree(q);
     if (stream) (void)hipStreamDestroy(stream);
   }
-  template <class T>
-  int alloc(T **q, size_t cnt) {
+  int raw(void **q, size_t bytes, bool clear, std::vector<void *> &list) {
     void *r = nullptr;
-    const size_t bytes = std::max<size_t>(cnt, 1) * sizeof(T);
+    bytes = std::max<size_t>(bytes, 256);
     hipError_t e = hipMalloc(&r, bytes);
     if (e != hipSuccess)
       return set_err(XCS_ENOMEM, "hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(e));
-    allocs.push_back(r);
-    e = hipMemsetAsync(r, 0, bytes, stream);
-    if (e != hipSuccess) return set_err(XCS_EHIP, "hipMemsetAsync: %s", hipGetErrorString(e));
-    *q = (T *)r;
+    list.push_back(r);
+    if (clear) {
+      e = hipMemsetAsync(r, 0, bytes, stream);
+      if (e != hipSuccess) return set_err(XCS_EHIP, "hipMemsetAsync: %s", hipGetErrorString(e));
+    }
+    *q = r;
     return XCS_OK;
+  }
+  template <class T>
+  int alloc(T **q, size_t cnt) {
+    return raw((void **)q, cnt * sizeof(T), true, allocs);
+  }
+  template <class T>
+  int walloc(T **q, size_t cnt) {
+    return raw((void **)q, cnt * sizeof(T), true, wallocs);
+  }
+};
+
+// Bump allocator over one device allocation (256-byte aligned pieces).
+// take() returns nullptr once the capacity would be exceeded.
+struct Arena {
+  char *base = nullptr;
+  size_t used = 0, cap = 0;
+  static size_t up(size_t b) { return (b + 255) / 256 * 256; }
+  template <class T>
+  T *take(size_t cnt) {
+    const size_t b = up(std::max<size_t>(cnt, 1) * sizeof(T));
+    if (!base || used + b > cap) return nullptr;
+    T *r = (T *)(base + used);
+    used += b;
+    return r;
   }
 };
 
@@ -694,6 +778,64 @@ static int batch_occupancy(int block, int *per_cu, int *cus) {
   HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, (const void *)solve_batch_kernel,
                                                        block, 0));
   if (*per_cu <= 0) return set_err(XCS_EINVAL, "block size %d does not fit on a CU", block);
+  return XCS_OK;
+}
+
+// Team geometry and per-team workspace for c workgroups of `block` threads.
+static int batch_configure(xcs_batch *B, int c, int block) {
+  if (block % 64 || block < 64 || block > 1024)
+    return set_err(XCS_EINVAL, "block_size must be a multiple of 64 in [64, 1024]");
+  if (c < 1) return set_err(XCS_EINVAL, "team_size must be >= 1");
+  if (B->alg == ALG_CSMPSP) c = 1;  // see plan_alloc: multi-workgroup CSMPSP is switched off
+  int per_cu, cus;
+  XCS_TRY(batch_occupancy(block, &per_cu, &cus));
+  const int resident = per_cu * cus;
+  if (c > resident) return set_err(XCS_EINVAL, "team_size %d exceeds %d resident workgroups", c, resident);
+  int nteams = std::min(B->count, resident / c);
+  if (c > 1 && nteams >= 8) nteams -= nteams % 8;  // whole teams per XCD
+  HIP_TRY(hipStreamSynchronize(B->stream));
+  B->drop_graph();
+  B->drop_work();
+  B->c = c;
+  B->block = block;
+  B->nteams = nteams;
+  B->grid = nteams * c;
+  const size_t T = (size_t)nteams, nn = (size_t)B->n_max, ml = (size_t)B->lda_max;
+  float *f_n, *f_m, *part;
+  int *i_n, *cnt, *wgc;
+  float2 *cw = nullptr;
+  XCS_TRY(B->walloc(&f_n, T * 4 * nn));
+  XCS_TRY(B->walloc(&i_n, T * 3 * nn));
+  XCS_TRY(B->walloc(&f_m, T * 3 * ml));
+  XCS_TRY(B->walloc(&cnt, T * ((size_t)B->bins_max + 64)));
+  XCS_TRY(B->walloc(&wgc, T * ((size_t)c + 64)));
+  XCS_TRY(B->walloc(&part, T * MAXCH * ml));
+  if (B->any_dct) XCS_TRY(B->walloc(&cw, T * 3 * nn));
+  std::vector<XWork> hw(T);
+  for (size_t t = 0; t < T; t++) {
+    XWork &W = hw[t];
+    xset(W.thres, f_n + (t * 4 + 0) * nn);
+    xset(W.grad, f_n + (t * 4 + 1) * nn);
+    xset(W.gprev, f_n + (t * 4 + 2) * nn);
+    xset(W.gprev_thres, f_n + (t * 4 + 3) * nn);
+    xset(W.bins, i_n + (t * 3 + 0) * nn);
+    xset(W.bins2, i_n + (t * 3 + 1) * nn);
+    xset(W.supp, i_n + (t * 3 + 2) * nn);
+    xset(W.resid, f_m + (t * 3 + 0) * ml);
+    xset(W.resid_update, f_m + (t * 3 + 1) * ml);
+    xset(W.resid2, f_m + (t * 3 + 2) * ml);
+    xset(W.counters, cnt + t * ((size_t)B->bins_max + 64));
+    xset(W.wg_counts, wgc + t * ((size_t)c + 64));
+    xset(W.partials, part + t * MAXCH * ml);
+    xset(W.cwork, cw ? cw + t * 3 * nn : nullptr);
+  }
+  XCS_TRY(B->walloc(&B->dW, T));
+  HIP_TRY(hipMemcpyAsync(B->dW, hw.data(), sizeof(XWork) * T, hipMemcpyHostToDevice, B->stream));
+  XCS_TRY(B->walloc(&B->slots, T));
+  XCS_TRY(B->walloc(&B->queue, 64));
+  XCS_TRY(B->walloc(&B->red, T * 2 * RED_SLOTS * (size_t)c));
+  XCS_TRY(B->walloc(&B->bar, T * BAR_WORDS));
+  HIP_TRY(hipStreamSynchronize(B->stream));  // hw goes out of scope
   return XCS_OK;
 }
 
@@ -715,82 +857,29 @@ static int batch_generate(xcs_batch *B, int alg, int count, const int *ens, cons
   HIP_TRY(hipEventCreate(&e0));
   HIP_TRY(hipEventCreate(&e1));
   HIP_TRY(hipEventRecord(e0, B->stream));
+  B->alg = alg;
   B->count = count;
-  B->c = o->team_size > 0 ? o->team_size : 1;
-  if (alg == ALG_CSMPSP) B->c = 1;  // see plan_alloc: multi-workgroup CSMPSP is switched off
-  B->block = o->block_size > 0 ? o->block_size : 256;
-  if (B->block % 64 || B->block > 1024) return set_err(XCS_EINVAL, "block_size must be a multiple of 64 <= 1024");
-  int per_cu, cus;
-  XCS_TRY(batch_occupancy(B->block, &per_cu, &cus));
-  const int resident = per_cu * cus;
-  if (B->c > resident) return set_err(XCS_EINVAL, "team_size %d exceeds %d resident workgroups", B->c, resident);
-  int nteams = std::min(count, resident / B->c);
-  if (B->c > 1 && nteams >= 8) nteams -= nteams % 8;  // whole teams per XCD
-  B->nteams = nteams;
-  B->grid = nteams * B->c;
-  int n_max = 1, lda_max = 64, bins_max = 1000;
   for (int i = 0; i < count; i++) {
-    n_max = std::max(n_max, n[i]);
-    lda_max = std::max(lda_max, round_up(m[i], 64));
-    bins_max = std::max(bins_max, o->num_bins > 0 ? o->num_bins : std::max(n[i] / 20, 1000));
+    B->n_max = std::max(B->n_max, n[i]);
+    B->lda_max = std::max(B->lda_max, round_up(m[i], 64));
+    B->bins_max = std::max(B->bins_max, o->num_bins > 0 ? o->num_bins : std::max(n[i] / 20, 1000));
+    B->any_dct |= ens[i] == ENS_DCT;
   }
-  bool any_dct = false;
-  for (int i = 0; i < count; i++) any_dct |= ens[i] == ENS_DCT;
-  // per-team workspace
-  std::vector<XWork> hw((size_t)nteams);
-  {
-    const size_t T = (size_t)nteams, nn = (size_t)n_max, ml = (size_t)lda_max;
-    float *f_n, *f_m, *part = nullptr;
-    int *i_n, *cnt, *wgc;
-    float2 *cw = nullptr;
-    XCS_TRY(B->alloc(&f_n, T * 4 * nn));
-    XCS_TRY(B->alloc(&i_n, T * 3 * nn));
-    XCS_TRY(B->alloc(&f_m, T * 3 * ml));
-    XCS_TRY(B->alloc(&cnt, T * ((size_t)bins_max + 64)));
-    XCS_TRY(B->alloc(&wgc, T * ((size_t)B->c + 64)));
-    XCS_TRY(B->alloc(&part, T * MAXCH * ml));
-    if (any_dct) XCS_TRY(B->alloc(&cw, T * 3 * nn));
-    for (size_t t = 0; t < T; t++) {
-      XWork &W = hw[t];
-      xset(W.thres, f_n + (t * 4 + 0) * nn);
-      xset(W.grad, f_n + (t * 4 + 1) * nn);
-      xset(W.gprev, f_n + (t * 4 + 2) * nn);
-      xset(W.gprev_thres, f_n + (t * 4 + 3) * nn);
-      xset(W.bins, i_n + (t * 3 + 0) * nn);
-      xset(W.bins2, i_n + (t * 3 + 1) * nn);
-      xset(W.supp, i_n + (t * 3 + 2) * nn);
-      xset(W.resid, f_m + (t * 3 + 0) * ml);
-      xset(W.resid_update, f_m + (t * 3 + 1) * ml);
-      xset(W.resid2, f_m + (t * 3 + 2) * ml);
-      xset(W.counters, cnt + t * ((size_t)bins_max + 64));
-      xset(W.wg_counts, wgc + t * ((size_t)B->c + 64));
-      xset(W.partials, part + t * MAXCH * ml);
-      xset(W.cwork, cw ? cw + t * 3 * nn : nullptr);
-    }
-    XCS_TRY(B->alloc(&B->dW, T));
-    HIP_TRY(hipMemcpyAsync(B->dW, hw.data(), sizeof(XWork) * T, hipMemcpyHostToDevice, B->stream));
-    XCS_TRY(B->alloc(&B->teamP, T));
-    XCS_TRY(B->alloc(&B->slots, T));
-    XCS_TRY(B->alloc(&B->queue, 1));
-    XCS_TRY(B->alloc(&B->red, T * 2 * RED_SLOTS * (size_t)B->c));
-    XCS_TRY(B->alloc(&B->bar, T * BAR_WORDS));
-  }
-  // instances
-  B->hP.resize((size_t)count);
+  XCS_TRY(batch_configure(B, o->team_size > 0 ? o->team_size : 1,
+                          o->block_size > 0 ? o->block_size : 256));
+
+  // descriptors and host-side draws (smv structure, dct rows, planted vectors)
+  std::vector<XProblem> hp((size_t)count);
   B->x_true.resize((size_t)count);
   B->ks.assign(k, k + count);
   B->last.resize((size_t)count);
-  // helper problem for y = A x (team 0's workspace, one workgroup)
-  XProblem *d_tmp;
-  XCS_TRY(B->alloc(&d_tmp, 1));
-  double *tmp_red;
-  unsigned *tmp_bar;
-  XCS_TRY(B->alloc(&tmp_red, 2 * RED_SLOTS * 2));
-  XCS_TRY(B->alloc(&tmp_bar, BAR_WORDS));
-  std::vector<std::pair<int, const float2 *>> tw_cache;
-  int maxiter_default = 0;
+  std::vector<std::vector<int>> supp((size_t)count), smv_rows((size_t)count), dct_rows((size_t)count);
+  std::vector<std::vector<float>> smv_vals((size_t)count);
+  int maxiter_max = 0;
+  size_t small_bytes = 0, big_bytes = 0;
+  std::vector<int> tw_n;
   for (int i = 0; i < count; i++) {
-    XProblem &P = B->hP[i];
+    XProblem &P = hp[i];
     memset(&P, 0, sizeof(P));
     P.alg = alg;
     P.ens = ens[i];
@@ -798,39 +887,97 @@ static int batch_generate(xcs_batch *B, int alg, int count, const int *ens, cons
     P.n = n[i];
     P.k = k[i];
     P.p = ens[i] == ENS_SMV ? p[i] : 0;
+    P.nz = P.n * P.p;
     P.lda = round_up(m[i], 64);
     P.num_bins = o->num_bins > 0 ? o->num_bins : std::max(n[i] / 20, 1000);
     P.maxiter = o->maxiter > 0 ? o->maxiter : ((alg == ALG_HTP || alg == ALG_CSMPSP) ? 300 : 5000);
-    maxiter_default = std::max(maxiter_default, P.maxiter);
+    maxiter_max = std::max(maxiter_max, P.maxiter);
     P.restart = o->restart;
     P.tol = o->tol_reference_default ? (float)(10 ^ (-4)) : o->tol;
     P.log2n = ens[i] == ENS_DCT ? ilog2_exact(n[i]) : 0;
-    const uint32_t seed = o->seed, key1 = inst ? inst[i] : (uint32_t)i;
-    float *y, *x;
-    XState *st;
-    XCS_TRY(B->alloc(&y, (size_t)P.lda));
-    XCS_TRY(B->alloc(&x, (size_t)P.n));
-    XCS_TRY(B->alloc(&st, 1));
+    const uint32_t key1 = inst ? inst[i] : (uint32_t)i;
+    host_sparse_vector(P.n, P.k, o->vec_distribution, o->seed, key1, B->x_true[i], supp[i]);
+    if (P.ens == ENS_SMV) XCS_TRY(host_smv(P.m, P.n, P.p, o->matrix_ensemble, o->seed, key1,
+                                          smv_rows[i], smv_vals[i]));
+    if (P.ens == ENS_DCT) host_subset(P.n, P.m, o->seed, key1, STREAM_DCT_ROWS, dct_rows[i]);
+    // small arena: y, x_true, planted support, smv/dct structure
+    small_bytes += Arena::up(4 * (size_t)P.lda) + Arena::up(4 * (size_t)P.n) +
+                   Arena::up(4 * std::max<size_t>(P.k, 1));
+    if (P.ens == ENS_SMV)
+      small_bytes += Arena::up(4 * ((size_t)P.n + 1)) + Arena::up(4 * ((size_t)P.m + 1)) +
+                     4 * Arena::up(4 * (size_t)P.nz);
+    if (P.ens == ENS_DCT) {
+      small_bytes += Arena::up(4 * (size_t)P.m);
+      if (std::find(tw_n.begin(), tw_n.end(), P.n) == tw_n.end()) {
+        tw_n.push_back(P.n);
+        small_bytes += Arena::up(16 * (size_t)P.n);
+      }
+    }
+    if (P.ens == ENS_GEN) big_bytes += Arena::up(4 * (size_t)P.lda * P.n);
+    B->xs_len += (size_t)P.n;
+  }
+  B->budget = maxiter_max + 4;
+
+  // queue order: largest instances first (work-queue load balance)
+  std::vector<int> order((size_t)count);
+  for (int i = 0; i < count; i++) order[i] = i;
+  auto cost = [&](int i) {
+    const XProblem &P = hp[i];
+    return P.ens == ENS_GEN ? (double)P.lda * P.n : P.ens == ENS_SMV ? 2.0 * P.nz : 6.0 * P.n;
+  };
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost(a) > cost(b); });
+  B->order = order;
+  B->slot_of.assign((size_t)count, 0);
+  for (int q = 0; q < count; q++) B->slot_of[order[q]] = q;
+
+  Arena small, big;
+  small.cap = small_bytes;
+  big.cap = big_bytes;
+  XCS_TRY(B->raw((void **)&small.base, small_bytes, true, B->allocs));
+  if (big_bytes) XCS_TRY(B->raw((void **)&big.base, big_bytes, false, B->allocs));
+#define XCS_ARENA_OK(ptr) \
+  if (!(ptr)) return set_err(XCS_EINVAL, "internal: instance arena overflow")
+  XCS_TRY(B->alloc(&B->xs, B->xs_len));
+  XCS_TRY(B->alloc(&B->sts, (size_t)count));
+  std::vector<std::pair<int, float2 *>> tw_cache;
+  std::vector<size_t> xt_off((size_t)count);
+  std::vector<const float *> xt_ptr((size_t)count);
+  std::vector<const int *> sp_ptr((size_t)count);
+  B->xoff.assign((size_t)count, 0);
+  B->hP.resize((size_t)count);
+  size_t xo = 0;
+  for (int q = 0; q < count; q++) {
+    const int i = order[q];
+    XProblem P = hp[i];
+    B->xoff[q] = xo;
+    xset(P.x, B->xs + xo);
+    xo += (size_t)P.n;
+    xset(P.st, B->sts + q);
+    float *y = small.take<float>(P.lda);
+    float *xt = small.take<float>(P.n);
+    int *sp = small.take<int>(P.k);
+    XCS_ARENA_OK(y && xt && sp);
     xset(P.y, y);
-    xset(P.x, x);
-    xset(P.st, st);
+    HIP_TRY(hipMemcpyAsync(xt, B->x_true[i].data(), 4 * (size_t)P.n, hipMemcpyHostToDevice, B->stream));
+    if (P.k) HIP_TRY(hipMemcpyAsync(sp, supp[i].data(), 4 * (size_t)P.k, hipMemcpyHostToDevice, B->stream));
+    xt_ptr[q] = xt;
+    sp_ptr[q] = sp;
     if (P.ens == ENS_GEN) {
-      float *A;
-      XCS_TRY(B->alloc(&A, (size_t)P.lda * P.n));
+      float *A = big.take<float>((size_t)P.lda * P.n);
+      XCS_ARENA_OK(A);
       const size_t total = (size_t)P.lda * P.n;
       const int blocks = (int)std::min<size_t>(65536, (total + 255) / 256);
       hipLaunchKernelGGL(gen_matrix_kernel, dim3(blocks), dim3(256), 0, B->stream, A, P.m, P.n,
-                         P.lda, seed, key1);
+                         P.lda, o->seed, inst ? inst[i] : (uint32_t)i);
       HIP_TRY(hipGetLastError());
       xset(P.A, A);
     } else if (P.ens == ENS_SMV) {
-      std::vector<int> rows, cp((size_t)P.n + 1), row_ptr((size_t)P.m + 1, 0), cols;
-      std::vector<float> vals, rvals;
-      XCS_TRY(host_smv(P.m, P.n, P.p, o->matrix_ensemble, seed, key1, rows, vals));
-      const int nz = P.n * P.p;
+      const std::vector<int> &rows = smv_rows[i];
+      const std::vector<float> &vals = smv_vals[i];
+      const int nz = P.nz;
+      std::vector<int> cp((size_t)P.n + 1), row_ptr((size_t)P.m + 1, 0), cols((size_t)nz);
+      std::vector<float> rvals((size_t)nz);
       for (int j = 0; j <= P.n; j++) cp[j] = j * P.p;
-      cols.resize((size_t)nz);
-      rvals.resize((size_t)nz);
       for (int e = 0; e < nz; e++) row_ptr[rows[e] + 1]++;
       for (int r = 0; r < P.m; r++) row_ptr[r + 1] += row_ptr[r];
       std::vector<int> fill(row_ptr.begin(), row_ptr.end() - 1);
@@ -840,86 +987,46 @@ static int batch_generate(xcs_batch *B, int alg, int count, const int *ens, cons
           cols[pos] = j;
           rvals[pos] = vals[e];
         }
-      int *d_cp, *d_r, *d_rp, *d_c;
-      float *d_v, *d_rv;
-      XCS_TRY(B->alloc(&d_cp, (size_t)P.n + 1));
-      XCS_TRY(B->alloc(&d_r, (size_t)nz));
-      XCS_TRY(B->alloc(&d_v, (size_t)nz));
-      XCS_TRY(B->alloc(&d_rp, (size_t)P.m + 1));
-      XCS_TRY(B->alloc(&d_c, (size_t)nz));
-      XCS_TRY(B->alloc(&d_rv, (size_t)nz));
-      HIP_TRY(hipMemcpyAsync(d_cp, cp.data(), sizeof(int) * (P.n + 1), hipMemcpyHostToDevice, B->stream));
-      HIP_TRY(hipMemcpyAsync(d_r, rows.data(), sizeof(int) * nz, hipMemcpyHostToDevice, B->stream));
-      HIP_TRY(hipMemcpyAsync(d_v, vals.data(), sizeof(float) * nz, hipMemcpyHostToDevice, B->stream));
-      HIP_TRY(hipMemcpyAsync(d_rp, row_ptr.data(), sizeof(int) * (P.m + 1), hipMemcpyHostToDevice, B->stream));
-      HIP_TRY(hipMemcpyAsync(d_c, cols.data(), sizeof(int) * nz, hipMemcpyHostToDevice, B->stream));
-      HIP_TRY(hipMemcpyAsync(d_rv, rvals.data(), sizeof(float) * nz, hipMemcpyHostToDevice, B->stream));
-      HIP_TRY(hipStreamSynchronize(B->stream));  // host vectors go out of scope
+      int *d_cp = small.take<int>((size_t)P.n + 1), *d_r = small.take<int>(nz);
+      float *d_v = small.take<float>(nz);
+      int *d_rp = small.take<int>((size_t)P.m + 1), *d_c = small.take<int>(nz);
+      float *d_rv = small.take<float>(nz);
+      XCS_ARENA_OK(d_cp && d_r && d_v && d_rp && d_c && d_rv);
+      HIP_TRY(hipMemcpyAsync(d_cp, cp.data(), 4 * cp.size(), hipMemcpyHostToDevice, B->stream));
+      HIP_TRY(hipMemcpyAsync(d_r, rows.data(), 4 * (size_t)nz, hipMemcpyHostToDevice, B->stream));
+      HIP_TRY(hipMemcpyAsync(d_v, vals.data(), 4 * (size_t)nz, hipMemcpyHostToDevice, B->stream));
+      HIP_TRY(hipMemcpyAsync(d_rp, row_ptr.data(), 4 * row_ptr.size(), hipMemcpyHostToDevice, B->stream));
+      HIP_TRY(hipMemcpyAsync(d_c, cols.data(), 4 * (size_t)nz, hipMemcpyHostToDevice, B->stream));
+      HIP_TRY(hipMemcpyAsync(d_rv, rvals.data(), 4 * (size_t)nz, hipMemcpyHostToDevice, B->stream));
+      HIP_TRY(hipStreamSynchronize(B->stream));  // the host vectors above go out of scope
       xset(P.col_ptr, d_cp);
       xset(P.rows, d_r);
       xset(P.vals, d_v);
       xset(P.row_ptr, d_rp);
       xset(P.cols, d_c);
       xset(P.rvals, d_rv);
-      P.nz = nz;
     } else {
-      std::vector<int> rows;
-      host_subset(P.n, P.m, seed, key1, STREAM_DCT_ROWS, rows);
-      int *d_rows;
-      XCS_TRY(B->alloc(&d_rows, (size_t)P.m));
-      HIP_TRY(hipMemcpyAsync(d_rows, rows.data(), sizeof(int) * P.m, hipMemcpyHostToDevice, B->stream));
-      HIP_TRY(hipStreamSynchronize(B->stream));
+      int *d_rows = small.take<int>(P.m);
+      XCS_ARENA_OK(d_rows);
+      HIP_TRY(hipMemcpyAsync(d_rows, dct_rows[i].data(), 4 * (size_t)P.m, hipMemcpyHostToDevice, B->stream));
       xset(P.dct_rows, d_rows);
-      const float2 *tw = nullptr;
-      for (auto &c : tw_cache)
-        if (c.first == P.n) tw = c.second;
+      float2 *tw = nullptr;
+      for (auto &e : tw_cache)
+        if (e.first == P.n) tw = e.second;
       if (!tw) {
-        float2 *t2;
-        XCS_TRY(B->alloc(&t2, 2 * (size_t)P.n));
+        tw = small.take<float2>(2 * (size_t)P.n);
+        XCS_ARENA_OK(tw);
         hipLaunchKernelGGL(dct_tables_kernel, dim3(std::min(4096, (P.n + 255) / 256)), dim3(256), 0,
-                           B->stream, t2, P.n);
+                           B->stream, tw, P.n);
         HIP_TRY(hipGetLastError());
-        tw_cache.push_back({P.n, t2});
-        tw = t2;
+        tw_cache.push_back({P.n, tw});
       }
       xset(P.twiddle, tw);
     }
-    // planted vector and y = A x with team 0's workspace
-    std::vector<int> supp;
-    host_sparse_vector(P.n, P.k, o->vec_distribution, seed, key1, B->x_true[i], supp);
-    XProblem Q = P;
-    attach_work(Q, hw[0]);
-    float *d_xt = Q.thres;  // team-0 scratch holds x_true during the product
-    HIP_TRY(hipMemcpyAsync(d_xt, B->x_true[i].data(), sizeof(float) * P.n, hipMemcpyHostToDevice, B->stream));
-    if (!supp.empty())
-      HIP_TRY(hipMemcpyAsync(Q.supp, supp.data(), sizeof(int) * supp.size(), hipMemcpyHostToDevice, B->stream));
-    HIP_TRY(hipMemcpyAsync(d_tmp, &Q, sizeof(XProblem), hipMemcpyHostToDevice, B->stream));
-    int transpose = 0, cnt = (int)supp.size();
-    const float *in = d_xt;
-    float *out = y;
-    void *args[] = {(void *)&d_tmp, (void *)&transpose, (void *)&in, (void *)&out, (void *)&cnt,
-                    (void *)&tmp_red, (void *)&tmp_bar};
-    HIP_TRY(hipLaunchCooperativeKernel((const void *)apply_coop_kernel, dim3(1), dim3(256), args, 0,
-                                       B->stream));
-    HIP_TRY(hipStreamSynchronize(B->stream));  // host x_true/supp reused next instance
+    B->hP[q] = P;
   }
-  B->budget = maxiter_default + 4;
-#ifdef XCS_PHASE_TIMING
-  XCS_TRY(B->alloc(&B->phase, 32));
-  for (XProblem &P : B->hP) xset(P.phase_ticks, B->phase);
-#endif
-  // largest instances first (work-queue load balance)
-  std::vector<int> order((size_t)count);
-  for (int i = 0; i < count; i++) order[i] = i;
-  auto cost = [&](int i) {
-    const XProblem &P = B->hP[i];
-    return P.ens == ENS_GEN ? (double)P.lda * P.n : P.ens == ENS_SMV ? 2.0 * P.nz : 6.0 * P.n;
-  };
-  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost(a) > cost(b); });
-  std::vector<XProblem> sorted((size_t)count);
-  for (int q = 0; q < count; q++) sorted[q] = B->hP[order[q]];
-  // keep the caller's indexing for results: remember the order
-  B->hP = sorted;
+#undef XCS_ARENA_OK
+  // planted vectors are now on the device; host copies are kept (queue order)
   {
     std::vector<std::vector<float>> xt2((size_t)count);
     std::vector<int> k2((size_t)count);
@@ -930,9 +1037,23 @@ static int batch_generate(xcs_batch *B, int alg, int count, const int *ens, cons
     B->x_true.swap(xt2);
     B->ks.swap(k2);
   }
-  B->order = order;
+#ifdef XCS_PHASE_TIMING
+  XCS_TRY(B->alloc(&B->phase, 32));
+  for (XProblem &P : B->hP) xset(P.phase_ticks, B->phase);
+#endif
   XCS_TRY(B->alloc(&B->dP, (size_t)count));
   HIP_TRY(hipMemcpyAsync(B->dP, B->hP.data(), sizeof(XProblem) * count, hipMemcpyHostToDevice, B->stream));
+  // y = A x_true for every instance, one workgroup each
+  const float **d_xt;
+  const int **d_sp;
+  XCS_TRY(B->alloc(&d_xt, (size_t)count));
+  XCS_TRY(B->alloc(&d_sp, (size_t)count));
+  HIP_TRY(hipMemcpyAsync(d_xt, xt_ptr.data(), sizeof(void *) * count, hipMemcpyHostToDevice, B->stream));
+  HIP_TRY(hipMemcpyAsync(d_sp, sp_ptr.data(), sizeof(void *) * count, hipMemcpyHostToDevice, B->stream));
+  const int ygrid = std::min(count, B->nteams);  // one team workspace per workgroup
+  hipLaunchKernelGGL(make_y_batch_kernel, dim3(ygrid), dim3(B->block), 0, B->stream, B->dP, count,
+                     B->dW, d_xt, d_sp);
+  HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(e1, B->stream));
   HIP_TRY(hipEventSynchronize(e1));
   HIP_TRY(hipEventElapsedTime(&B->gen_ms, e0, e1));
@@ -941,20 +1062,16 @@ static int batch_generate(xcs_batch *B, int alg, int count, const int *ens, cons
   return XCS_OK;
 }
 
-static int batch_run(xcs_batch *B, float *ms) {
-  for (const XProblem &P : B->hP) {
-    HIP_TRY(hipMemsetAsync(P.x, 0, sizeof(float) * P.n, B->stream));
-    HIP_TRY(hipMemsetAsync(P.st, 0, sizeof(XState), B->stream));
-  }
-  HIP_TRY(hipMemsetAsync(B->queue, 0, sizeof(int), B->stream));
-  HIP_TRY(hipMemsetAsync(B->bar, 0, sizeof(unsigned) * BAR_WORDS * B->nteams, B->stream));
-  hipEvent_t e0, e1;
-  HIP_TRY(hipEventCreate(&e0));
-  HIP_TRY(hipEventCreate(&e1));
-  HIP_TRY(hipEventRecord(e0, B->stream));
-  void *args[] = {(void *)&B->dP, (void *)&B->count, (void *)&B->queue, (void *)&B->teamP,
-                  (void *)&B->dW, (void *)&B->slots, (void *)&B->red, (void *)&B->bar,
-                  (void *)&B->c, (void *)&B->budget};
+// Reset, then solve: the two launches of one batched solve, on B->stream.
+static int batch_enqueue(xcs_batch *B) {
+  const int rgrid = (int)std::min<size_t>(4096, (B->xs_len + 255) / 256 + 1);
+  hipLaunchKernelGGL(batch_reset_kernel, dim3(rgrid), dim3(256), 0, B->stream, B->xs, B->xs_len,
+                     (int *)B->sts, sizeof(XState) / sizeof(int) * (size_t)B->count, B->queue,
+                     B->bar, (size_t)BAR_WORDS * B->nteams);
+  HIP_TRY(hipGetLastError());
+  void *args[] = {(void *)&B->dP, (void *)&B->count, (void *)&B->queue, (void *)&B->dW,
+                  (void *)&B->slots, (void *)&B->red, (void *)&B->bar, (void *)&B->c,
+                  (void *)&B->budget};
   if (B->c > 1) {
     HIP_TRY(hipLaunchCooperativeKernel((const void *)solve_batch_kernel, dim3(B->grid),
                                        dim3(B->block), args, 0, B->stream));
@@ -962,19 +1079,148 @@ static int batch_run(xcs_batch *B, float *ms) {
     HIP_TRY(hipLaunchKernel((const void *)solve_batch_kernel, dim3(B->grid), dim3(B->block), args,
                             0, B->stream));
   }
+  return XCS_OK;
+}
+
+// Single-workgroup teams: the reset and the persistent solve are captured
+// once into a HIP graph and replayed.  Multi-workgroup teams need the
+// cooperative launch (co-residency of a team's workgroups), which is issued
+// directly.
+static int batch_capture(xcs_batch *B) {
+  B->drop_graph();
+  if (B->c > 1) return XCS_OK;
+  hipGraph_t g = nullptr;
+  HIP_TRY(hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal));
+  const int rc = batch_enqueue(B);
+  const hipError_t e = hipStreamEndCapture(B->stream, &g);
+  if (rc != XCS_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return set_err(XCS_EHIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+  const hipError_t e2 = hipGraphInstantiate(&B->graph, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e2 != hipSuccess) {
+    B->graph = nullptr;
+    return set_err(XCS_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(e2));
+  }
+  return XCS_OK;
+}
+
+static int batch_run(xcs_batch *B, float *ms) {
+  if (B->c == 1 && !B->graph) XCS_TRY(batch_capture(B));
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  HIP_TRY(hipEventRecord(e0, B->stream));
+  if (B->graph) {
+    HIP_TRY(hipGraphLaunch(B->graph, B->stream));
+  } else {
+    XCS_TRY(batch_enqueue(B));
+  }
   HIP_TRY(hipEventRecord(e1, B->stream));
   HIP_TRY(hipEventSynchronize(e1));
   HIP_TRY(hipEventElapsedTime(&B->last_ms, e0, e1));
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  for (int q = 0; q < B->count; q++)
-    HIP_TRY(hipMemcpyAsync(&B->last[q], B->hP[q].st, sizeof(XState), hipMemcpyDeviceToHost, B->stream));
+  HIP_TRY(hipMemcpyAsync(B->last.data(), B->sts, sizeof(XState) * B->count, hipMemcpyDeviceToHost,
+                         B->stream));
   std::vector<unsigned> bars((size_t)BAR_WORDS * B->nteams);
   HIP_TRY(hipMemcpyAsync(bars.data(), B->bar, sizeof(unsigned) * bars.size(), hipMemcpyDeviceToHost, B->stream));
   HIP_TRY(hipStreamSynchronize(B->stream));
   for (int t = 0; t < B->nteams; t++)
     if (bars[(size_t)t * BAR_WORDS + 2]) return set_err(XCS_EHIP, "team barrier timed out (results invalid)");
   if (ms) *ms = B->last_ms;
+  return XCS_OK;
+}
+
+// ---------------------------------------------------------------- autotune --
+// Candidate team geometries are timed on the batch itself with every
+// instance capped at `probe_iters` iterations (0: full solves); the fastest
+// is kept.  Results are cached in a text file, one line per batch signature:
+//   <signature> <team_size> <block> <probe_ms>
+static std::string batch_signature(const xcs_batch *B, int probe_iters) {
+  static const char *names[] = {"NIHT", "HTP", "CSMPSP", "CGIHT"};
+  static const char *ens_names[] = {"gen", "smv", "dct"};
+  std::string s = std::string(names[B->alg]) + ":count=" + std::to_string(B->count);
+  for (int e = 0; e < 3; e++) {
+    int cnt = 0, nmin = 1 << 30, nmax = 0;
+    double bytes = 0.0;
+    for (const XProblem &P : B->hP)
+      if (P.ens == e) {
+        cnt++;
+        nmin = std::min(nmin, P.n);
+        nmax = std::max(nmax, P.n);
+        bytes += P.ens == ENS_GEN ? 4.0 * P.lda * P.n : P.ens == ENS_SMV ? 8.0 * P.nz : 24.0 * P.n;
+      }
+    if (cnt)
+      s += std::string(":") + ens_names[e] + "=" + std::to_string(cnt) + "/" + std::to_string(nmin) +
+           "-" + std::to_string(nmax) + "/2^" + std::to_string((int)std::lround(std::log2(bytes)));
+  }
+  return s + ":probe=" + std::to_string(probe_iters);
+}
+
+static int batch_tune(xcs_batch *B, int probe_iters, const char *cache_path, int *team_out,
+                      int *block_out, float *ms_out) {
+  const std::string sig = batch_signature(B, probe_iters);
+  if (cache_path && *cache_path) {
+    FILE *f = fopen(cache_path, "r");
+    if (f) {
+      char key[1024];
+      int c, b;
+      float t;
+      bool hit = false;
+      while (fscanf(f, "%1023s %d %d %f", key, &c, &b, &t) == 4)
+        if (sig == key) {
+          hit = true;
+          *team_out = c;
+          *block_out = b;
+          *ms_out = t;
+        }
+      fclose(f);
+      if (hit) {
+        XCS_TRY(batch_configure(B, *team_out, *block_out));
+        return XCS_OK;
+      }
+    }
+  }
+  const int saved_budget = B->budget;
+  const int budget = probe_iters > 0 ? probe_iters : saved_budget;
+  static const int blocks[] = {256, 512, 1024};
+  static const int teams[] = {1, 2, 4, 8};
+  float best = 1e30f;
+  int best_c = B->c, best_b = B->block;
+  for (int bi = 0; bi < 3; bi++)
+    for (int ti = 0; ti < 4; ti++) {
+      const int c = teams[ti], blk = blocks[bi];
+      if (c > 1 && B->alg == ALG_CSMPSP) continue;
+      int per_cu, cus;
+      XCS_TRY(batch_occupancy(blk, &per_cu, &cus));
+      if (c > 1 && per_cu * cus / c < 8) continue;
+      XCS_TRY(batch_configure(B, c, blk));
+      B->budget = budget;
+      float ms = 0.f;
+      const int rc = batch_run(B, &ms);
+      B->budget = saved_budget;
+      if (rc != XCS_OK) return rc;
+      if (ms < best) {
+        best = ms;
+        best_c = c;
+        best_b = blk;
+      }
+    }
+  B->budget = saved_budget;
+  XCS_TRY(batch_configure(B, best_c, best_b));
+  *team_out = best_c;
+  *block_out = best_b;
+  *ms_out = best;
+  if (cache_path && *cache_path) {
+    FILE *f = fopen(cache_path, "a");
+    if (f) {
+      fprintf(f, "%s %d %d %.4f\n", sig.c_str(), best_c, best_b, best);
+      fclose(f);
+    }
+  }
   return XCS_OK;
 }
 
@@ -1373,14 +1619,13 @@ int xcs_batch_results(xcs_batch *B, float *norms, int *iterations, int *stop_rea
   if (!B) return set_err(XCS_EINVAL, "NULL batch");
   HIP_TRY(hipSetDevice(B->dev));
   std::vector<float> xo;
+  if (norms) {
+    xo.resize(B->xs_len);
+    HIP_TRY(hipMemcpy(xo.data(), B->xs, sizeof(float) * B->xs_len, hipMemcpyDeviceToHost));
+  }
   for (int q = 0; q < B->count; q++) {
     const int i = B->order[q];
-    const XProblem &P = B->hP[q];
-    if (norms) {
-      xo.resize((size_t)P.n);
-      HIP_TRY(hipMemcpy(xo.data(), P.x, sizeof(float) * P.n, hipMemcpyDeviceToHost));
-      host_norms(B->x_true[q], xo.data(), B->ks[q], norms + 3 * (size_t)i, nullptr);
-    }
+    if (norms) host_norms(B->x_true[q], xo.data() + B->xoff[q], B->ks[q], norms + 3 * (size_t)i, nullptr);
     if (iterations) iterations[i] = B->last[q].iter;
     if (stop_reason) stop_reason[i] = B->last[q].stop_reason;
     if (conv_rate) conv_rate[i] = conv_rate_results(B->last[q]);
@@ -1391,12 +1636,25 @@ int xcs_batch_results(xcs_batch *B, float *norms, int *iterations, int *stop_rea
 int xcs_batch_x(xcs_batch *B, int idx, float *x_out, float *x_true) {
   if (!B || idx < 0 || idx >= B->count) return set_err(XCS_EINVAL, "bad batch index");
   HIP_TRY(hipSetDevice(B->dev));
-  int q = 0;
-  while (B->order[q] != idx) q++;
+  const int q = B->slot_of[idx];
   const XProblem &P = B->hP[q];
-  if (x_out) HIP_TRY(hipMemcpy(x_out, P.x, sizeof(float) * P.n, hipMemcpyDeviceToHost));
+  if (x_out) HIP_TRY(hipMemcpy(x_out, B->xs + B->xoff[q], sizeof(float) * P.n, hipMemcpyDeviceToHost));
   if (x_true) memcpy(x_true, B->x_true[q].data(), sizeof(float) * P.n);
   return XCS_OK;
+}
+
+int xcs_batch_configure(xcs_batch *B, int team_size, int block_size) {
+  if (!B) return set_err(XCS_EINVAL, "NULL batch");
+  HIP_TRY(hipSetDevice(B->dev));
+  return batch_configure(B, team_size, block_size);
+}
+
+int xcs_batch_tune(xcs_batch *B, int probe_iters, const char *cache_path, int *team_size,
+                   int *block_size, float *probe_ms) {
+  if (!B || !team_size || !block_size || !probe_ms) return set_err(XCS_EINVAL, "NULL argument");
+  if (probe_iters < 0) return set_err(XCS_EINVAL, "probe_iters must be >= 0");
+  HIP_TRY(hipSetDevice(B->dev));
+  return batch_tune(B, probe_iters, cache_path, team_size, block_size, probe_ms);
 }
 
 int xcs_batch_info(xcs_batch *B, int *teams, int *team_size, int *block, float *gen_ms) {

@@ -1,0 +1,181 @@
+"""Phase-transition sweeps on the batched engine.
+
+The reference measures a phase transition with serial gaga_cs calls: a
+bisection on k for each (delta, n) that runs 10 problems per probe
+(GAGA/CGIHTcode/shared/tests_delta_fixed.m:48-137), followed by a logistic
+fit of the success fraction against rho = k/m
+(GAGA/CGIHTcode/shared/calc_logit.m, logit_model.m, logit_model_error.m).
+Here every probe of every delta runs as one batch (one persistent launch per
+bisection round), and the fixed (delta, rho) grid of a sweep is a single
+batch.
+"""
+import math
+
+import numpy as np
+
+from ._capi import Batch
+
+SUCCESS_TOL = 1e-3  # relative l_inf error (tests_delta_fixed.m:17-23)
+
+
+def grid_specs(ensembles=("gen", "smv"), ns=(1024, 2048, 4096, 8192),
+               deltas=None, rhos=None, seeds=2, p=7, first_instance=0):
+    """Instances of a (delta, rho) grid: m = ceil(delta n), k = ceil(rho m).
+
+    The default is the 4096-instance sweep: gen and smv (p = 7), n in
+    {1024, 2048, 4096, 8192}, delta = linspace(0.05, 0.95, 16),
+    rho = linspace(0.05, 0.5, 16), two seeds per point.  Instance ids are
+    consecutive from first_instance (the generator key)."""
+    deltas = np.linspace(0.05, 0.95, 16) if deltas is None else np.asarray(deltas)
+    rhos = np.linspace(0.05, 0.5, 16) if rhos is None else np.asarray(rhos)
+    specs = []
+    inst = first_instance
+    for ens in ensembles:
+        for n in ns:
+            for d in deltas:
+                m = int(math.ceil(d * n - 1e-9))
+                for r in rhos:
+                    k = max(1, int(math.ceil(r * m - 1e-9)))
+                    for _ in range(seeds):
+                        s = dict(ens=ens, n=int(n), m=m, k=k, instance=inst,
+                                 delta=float(d), rho=float(r))
+                        if ens == "smv":
+                            s["p"] = p
+                        specs.append(s)
+                        inst += 1
+    return specs
+
+
+def run_batch(alg, specs, opts, tune=0, cache=None):
+    """Solve every spec in one batch; returns (success bool array, results dict,
+    device ms, batch info)."""
+    B = Batch(alg, specs, opts)
+    try:
+        if tune:
+            B.tune(probe_iters=tune, cache=cache)
+        ms = B.solve()
+        r = B.results()
+        info = B.info()
+        info["bytes"] = B.bytes()
+    finally:
+        B.close()
+    return r["norms"][:, 2] < SUCCESS_TOL, r, ms, info
+
+
+def logit_model(rho, b):
+    """Success probability model (logit_model.m): 1 / (1 + exp(-b0 (1 - b1 rho)))."""
+    rho = np.asarray(rho, np.float64)
+    z = np.clip(-b[0] * (1.0 - b[1] * rho), -700.0, 700.0)
+    return 1.0 / (1.0 + np.exp(z))
+
+
+def logit_model_error(rho, success, num_tests, b):
+    """Weighted l1 misfit of the model (logit_model_error.m)."""
+    num_tests = np.asarray(num_tests, np.float64)
+    success = np.asarray(success, np.float64)
+    y = logit_model(rho, b)
+    return float(num_tests @ np.abs(success - y)) / max(1.0, float(num_tests @ success))
+
+
+def calc_logit(k, m, success, num_tests):
+    """Fit (b0, b1) of the logistic model to success fractions at k/m
+    (calc_logit.m): b1 starts from the best step function, then a bounded
+    local search is run from 200 slopes in [-m, -1/max(rho)] and the
+    best fit is kept (the middle one on ties); Nelder-Mead inside the
+    bounds stands in for fmincon.  rho at 50% success is 1/b1."""
+    from scipy.optimize import minimize
+    rho = np.asarray(k, np.float64) / np.asarray(m, np.float64)
+    m0 = float(np.asarray(m).flat[0])
+    f = lambda b: logit_model_error(rho, success, num_tests, b)
+    err0 = np.array([f([4 * m0 / r, 1 / r]) for r in rho])
+    ind = np.flatnonzero(err0 == err0.min())
+    b1 = 1.0 / rho[ind[0]] if len(ind) == 1 else 1.0 / rho[ind].mean()
+    r_tmp = 1.0 / b1
+    slope = np.linspace(-m0, -1.0 / rho.max(), 200)
+    b0_list = -4.0 * slope * r_tmp
+    lo = [b0_list.min(), 1.0]
+    hi = [b0_list.max(), m0]
+    errs, bs = [], []
+    for b0 in b0_list:
+        res = minimize(f, [b0, b1], method="Nelder-Mead", bounds=list(zip(lo, hi)),
+                       options=dict(xatol=1e-6, fatol=1e-6, maxfev=5000))
+        errs.append(f(res.x))
+        bs.append(res.x)
+    errs = np.array(errs)
+    ind = np.flatnonzero(errs == errs.min())
+    return np.asarray(bs[ind[(len(ind) + 1) // 2 - 1]], np.float64)
+
+
+def transition_bisect(alg, ens, m_list, n, opts_fn, p=7, tests=10, first_instance=1 << 20,
+                      success_frac=0.95, failure_frac=0.05, solve=None):
+    """Bracket the transition in k for every m of m_list at once
+    (tests_delta_fixed.m:48-137): k_min is the largest k seen with success
+    fraction > success_frac, k_max the smallest with fraction < failure_frac.
+    Each bisection round solves `tests` problems per unfinished m in one
+    batch.  opts_fn() gives the options; solve(specs) may replace the GPU
+    batch (it returns a success array).  Returns {m: (k_min, k_max)}."""
+    solve = solve or (lambda specs: run_batch(alg, specs, opts_fn())[0])
+    state = {}
+    for m in m_list:
+        state[m] = dict(k_min=1, k_up=m - 1, k_max=m - 1, phase=0)
+        state[m]["mid"] = round((1 + m - 1) / 2)
+    inst = first_instance
+    while True:
+        active = []
+        for m, s in state.items():
+            tol = math.ceil(math.sqrt(m) / 10)
+            if s["phase"] == 0 and s["k_up"] - s["k_min"] <= tol:
+                s["phase"] = 1
+                s["k_down"] = s["k_min"]
+                s["mid"] = round((s["k_max"] + s["k_down"]) / 2)
+            if s["phase"] == 1 and s["k_max"] - s["k_down"] <= tol:
+                s["phase"] = 2
+            if s["phase"] < 2:
+                active.append(m)
+        if not active:
+            break
+        specs = []
+        for m in active:
+            for _ in range(tests):
+                sp = dict(ens=ens, n=n, m=m, k=max(1, state[m]["mid"]), instance=inst)
+                if ens == "smv":
+                    sp["p"] = p
+                specs.append(sp)
+                inst += 1
+        ok = np.asarray(solve(specs), bool)
+        for j, m in enumerate(active):
+            frac = ok[j * tests:(j + 1) * tests].mean()
+            s = state[m]
+            if s["phase"] == 0:
+                if frac > success_frac:
+                    s["k_min"] = s["mid"]
+                elif frac < success_frac:
+                    s["k_up"] = s["mid"]
+                    if frac < failure_frac:
+                        s["k_max"] = s["mid"]
+                s["mid"] = round((s["k_min"] + s["k_up"]) / 2)
+            else:
+                if frac < failure_frac:
+                    s["k_max"] = s["mid"]
+                elif frac > failure_frac:
+                    s["k_down"] = s["mid"]
+                s["mid"] = round((s["k_max"] + s["k_down"]) / 2)
+    return {m: (s["k_min"], s["k_max"]) for m, s in state.items()}
+
+
+def transition_table(specs, success):
+    """Per (ensemble, n, delta): the logistic fit's 50% point rho* = 1/b1 and
+    the raw success fractions per rho."""
+    groups = {}
+    for s, ok in zip(specs, success):
+        key = (s["ens"], s["n"], s["m"])
+        groups.setdefault(key, {}).setdefault(s["k"], []).append(bool(ok))
+    out = []
+    for (ens, n, m), by_k in sorted(groups.items()):
+        ks = np.array(sorted(by_k))
+        frac = np.array([np.mean(by_k[k]) for k in ks])
+        cnt = np.array([len(by_k[k]) for k in ks], np.float64)
+        b = calc_logit(ks, np.full(len(ks), m), frac, cnt) if len(ks) > 1 else None
+        out.append(dict(ens=ens, n=n, m=m, delta=m / n, k=ks.tolist(), success=frac.tolist(),
+                        rho_star=(1.0 / b[1]) if b is not None else None))
+    return out

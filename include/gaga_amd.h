@@ -115,8 +115,19 @@ typedef struct xcs_batch xcs_batch;
 int xcs_batch_create(const char *alg, int count, const int *ens, const int *k, const int *m,
                      const int *n, const int *p, const unsigned int *instance,
                      const xcs_options *opts, xcs_batch **batch);
-/* Solve every instance from x = 0 in one persistent launch; *ms = device time. */
+/* Solve every instance from x = 0 in one persistent launch; *ms = device time.
+ * With one workgroup per instance the reset and the launch are a captured HIP graph,
+ * replayed on every call. */
 int xcs_batch_solve(xcs_batch *batch, float *ms);
+/* Change the team geometry (workgroups per instance, threads per workgroup); results
+ * do not depend on it. */
+int xcs_batch_configure(xcs_batch *batch, int team_size, int block_size);
+/* Autotuner: time every candidate geometry (team 1/2/4/8 x block 256/512/1024) on this
+ * batch with each instance capped at probe_iters iterations (0: whole solves), keep the
+ * fastest.  cache_path (may be NULL) is a text file of "<signature> <team> <block> <ms>"
+ * lines: a hit skips the probes, a miss appends the result. */
+int xcs_batch_tune(xcs_batch *batch, int probe_iters, const char *cache_path, int *team_size,
+                   int *block_size, float *probe_ms);
 /* Per-instance results, caller indexing: norms[3*i..3*i+2] relative l1/l2/linf. */
 int xcs_batch_results(xcs_batch *batch, float *norms, int *iterations, int *stop_reason,
                       float *conv_rate);
