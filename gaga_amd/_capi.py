@@ -15,7 +15,9 @@ EXPORTS = (
     "xcs_batch_create", "xcs_batch_bytes", "xcs_batch_apply_timed", "xcs_batch_solve",
     "xcs_batch_results", "xcs_batch_configure", "xcs_batch_tune",
     "xcs_batch_x", "xcs_batch_info", "xcs_batch_destroy", "xcs_batch_phase_ticks",
-    "xcs_find_support_set", "xcs_apply", "xcs_generate_gen",
+    "xcs_find_support_set", "xcs_apply", "xcs_generate_gen", "xcs_gemm_tn",
+    "xcs_mrhs_create", "xcs_mrhs_solve", "xcs_mrhs_results", "xcs_mrhs_x",
+    "xcs_mrhs_gemm_timed", "xcs_mrhs_phase_ms", "xcs_mrhs_fp32_solve", "xcs_mrhs_destroy",
     "xcs_last_error", "xcs_version",
 )
 
@@ -71,6 +73,16 @@ def load():
         L.xcs_batch_create.argtypes = [_C, _I, _P, _P, _P, _P, _P, _P, _P, _P]
         L.xcs_batch_solve.argtypes = [_P, _P]
         L.xcs_batch_configure.argtypes = [_P, _I, _I]
+        L.xcs_gemm_tn.argtypes = [_C, _I, _I, _I, _P, _P, _P, _P, _P]
+        L.xcs_mrhs_create.argtypes = [_I, _I, _I, _I, _C, _P, _P]
+        L.xcs_mrhs_solve.argtypes = [_P, _P, _P]
+        L.xcs_mrhs_results.argtypes = [_P, _P, _P, _P]
+        L.xcs_mrhs_x.argtypes = [_P, _I, _P, _P, _P]
+        L.xcs_mrhs_gemm_timed.argtypes = [_P, _I, _C, _I, _P, _P]
+        L.xcs_mrhs_phase_ms.argtypes = [_P, _P]
+        L.xcs_mrhs_fp32_solve.argtypes = [_P, _I, _I, _I, _P, _P, _P]
+        L.xcs_mrhs_destroy.argtypes = [_P]
+        L.xcs_mrhs_destroy.restype = None
         L.xcs_batch_tune.argtypes = [_P, _I, _C, _P, _P, _P]
         L.xcs_batch_results.argtypes = [_P, _P, _P, _P, _P]
         L.xcs_batch_x.argtypes = [_P, _I, _P, _P]
@@ -333,6 +345,85 @@ class Batch:
     def close(self):
         if self._h:
             load().xcs_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def gemm_tn(fmt, A, B):
+    """C = q(A) q(B)^T on the MFMA GEMM; returns (C (M x N), A_q, B_q)."""
+    A = np.ascontiguousarray(A, np.float32)
+    B = np.ascontiguousarray(B, np.float32)
+    M, K = A.shape
+    N = B.shape[0]
+    C = np.zeros((N, M), np.float32)
+    Aq = np.zeros_like(A)
+    Bq = np.zeros_like(B)
+    _ck(load().xcs_gemm_tn(fmt.encode(), M, N, K, _p(A), _p(B), _p(C), _p(Aq), _p(Bq)))
+    return C.T.copy(), Aq, Bq
+
+
+class ManyRHS:
+    """One gen A shared by `count` signals, CGIHT in lock step (xcs_mrhs_*)."""
+
+    def __init__(self, m, n, k, count, precision="bf16", opts=None):
+        from . import _apply_options
+        self.m, self.n, self.k, self.count = m, n, k, count
+        o = _apply_options("CGIHT", "gen", n, opts)
+        h = ctypes.c_void_p()
+        _ck(load().xcs_mrhs_create(m, n, k, count, precision.encode(), ctypes.byref(o),
+                                   ctypes.byref(h)))
+        self._h = h
+
+    def solve(self):
+        ms = ctypes.c_float(0)
+        it = np.zeros(3, np.int32)
+        _ck(load().xcs_mrhs_solve(self._h, ctypes.byref(ms), _p(it)))
+        return ms.value, it
+
+    def results(self):
+        norms = np.zeros((self.count, 3), np.float32)
+        it = np.zeros(self.count, np.int32)
+        rr = np.zeros(self.count, np.float64)
+        _ck(load().xcs_mrhs_results(self._h, _p(norms), _p(it), _p(rr)))
+        return dict(norms=norms, iters=it, resid_rel=rr)
+
+    def x(self, s):
+        x = np.zeros(self.n, np.float32)
+        xt = np.zeros(self.n, np.float32)
+        y = np.zeros(self.m, np.float32)
+        _ck(load().xcs_mrhs_x(self._h, int(s), _p(x), _p(xt), _p(y)))
+        return x, xt, y
+
+    def fp32_solve(self, first, cnt, mode="device"):
+        """Signals [first, first + cnt) by the FP32 one-instance-per-launch engine:
+        (summed ms, norms (cnt x 3), iterations)."""
+        ms = ctypes.c_float(0)
+        norms = np.zeros((cnt, 3), np.float32)
+        it = np.zeros(cnt, np.int32)
+        _ck(load().xcs_mrhs_fp32_solve(self._h, int(first), int(cnt), 1 if mode == "plain" else 0,
+                                       ctypes.byref(ms), _p(norms), _p(it)))
+        return ms.value, norms, it
+
+    def phase_ms(self):
+        out = np.zeros(6, np.float32)
+        _ck(load().xcs_mrhs_phase_ms(self._h, _p(out)))
+        return out
+
+    def gemm_timed(self, transpose, fmt="bf16", reps=20):
+        ms = ctypes.c_float(0)
+        fl = ctypes.c_double(0)
+        _ck(load().xcs_mrhs_gemm_timed(self._h, int(transpose), fmt.encode(), int(reps),
+                                       ctypes.byref(ms), ctypes.byref(fl)))
+        return ms.value, fl.value
+
+    def close(self):
+        if self._h:
+            load().xcs_mrhs_destroy(self._h)
             self._h = None
 
     def __del__(self):

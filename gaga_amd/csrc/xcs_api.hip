@@ -1720,3 +1720,7 @@ int xcs_batch_phase_ticks(xcs_batch *B, unsigned long long *out16) {
 }
 
 }  // extern "C"
+
+// --------------------------------------------------- many right-hand sides --
+#include "xcs_mfma.h"
+#include "xcs_mrhs.h"

@@ -1,0 +1,221 @@
+// Matrix-core (MFMA) products for many right-hand sides, gfx950.
+//
+// C = A_op * B_op with both operands K-contiguous ("TN"):
+//   A_op: M x K, row r at A + r * lda_bytes
+//   B_op: K x N, column c at B + col(c) * ldb_bytes
+//   C:    C[col(c) * ldc + r] (fp32), optionally one slice per K split
+// col(c) = col_map ? col_map[c] : c for c < *ncols, so a product can run over
+// the active signals only.  Formats: BF16 (v_mfma_f32_32x32x16_bf16) and OCP
+// FP8 e4m3 (v_mfma_f32_32x32x64_f8f6f4, format code 0 for both operands);
+// accumulation in FP32.
+//
+// Tiling: 256 threads = 4 waves in a 2 x 2 layout, each wave a 64 x 64 tile
+// made of 2 x 2 MFMA 32 x 32 blocks; 128 x 128 block tile; 128 bytes of K per
+// stage (64 bf16 or 128 fp8), two LDS stages (72 KB) with the next stage
+// prefetched into registers while the current one is multiplied.
+//
+// MFMA operand placement used here: lane l holds row (l % 32) of the A block
+// and column (l % 32) of the B block, with the same run of K values (bytes
+// [16 * (l / 32), +16) of each 32-byte K slice for bf16, [32 * (l / 32), +32)
+// of each 64-byte slice for fp8) in both, so every product pairs equal K.
+// Result: accumulator v of lane l is C[8 (v / 4) + 4 (l / 32) + v % 4][l % 32].
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace xcs {
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x16_t __attribute__((ext_vector_type(16)));
+typedef int i32x8_t __attribute__((ext_vector_type(8)));
+typedef int i32x4_t __attribute__((ext_vector_type(4)));
+
+enum GemmFmt { FMT_BF16 = 0, FMT_FP8 = 1 };
+
+constexpr int GT_M = 128, GT_N = 128;  // block tile
+constexpr int GT_KB = 128;             // K bytes per stage
+constexpr int GT_ROW = GT_KB + 16;     // padded LDS row, bytes (conflict-free b128 reads)
+constexpr int GT_THREADS = 256;
+constexpr float FP8_MAX = 448.0f;      // largest finite OCP e4m3 value
+
+struct GemmArgs {
+  const uint8_t *A;
+  long lda_bytes;
+  const uint8_t *B;
+  long ldb_bytes;
+  float *C;
+  long ldc;           // floats
+  long split_stride;  // floats between K-split slices of C
+  int M, K, N;        // K in elements; N = columns the grid covers
+  int ksplit;
+  const int *ncols;      // device, nullable (then N)
+  const int *col_map;    // device, nullable (identity)
+  const float *row_scale;  // per output row, nullable
+  const float *col_scale;  // per physical column, nullable
+};
+
+__device__ __forceinline__ int gemm_esize(int fmt) { return fmt == FMT_BF16 ? 2 : 1; }
+
+template <int FMT>
+__global__ void __launch_bounds__(GT_THREADS) gemm_tn_kernel(GemmArgs g) {
+  __shared__ __attribute__((aligned(16))) uint8_t lds[2][2][GT_M * GT_ROW];
+  const int ncols = g.ncols ? *g.ncols : g.N;
+  const int m0 = blockIdx.x * GT_M, n0 = blockIdx.y * GT_N;
+  if (n0 >= ncols) return;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const long kbytes = (long)g.K * (FMT == FMT_BF16 ? 2 : 1);
+  const long kb_split = kbytes / g.ksplit;
+  const long kb0 = (long)blockIdx.z * kb_split;
+  const int nsteps = (int)(kb_split / GT_KB);
+
+  // global -> register assignment: 4 chunks of 16 bytes per thread per operand
+  const uint8_t *pa[4], *pb[4];
+  bool okb[4];
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const int row = (tid >> 3) + 32 * i;
+    pa[i] = g.A + (long)(m0 + row) * g.lda_bytes + kb0 + (tid & 7) * 16;
+    const int c = n0 + row;
+    okb[i] = c < ncols;
+    const int phys = okb[i] ? (g.col_map ? g.col_map[c] : c) : 0;
+    pb[i] = g.B + (long)phys * g.ldb_bytes + kb0 + (tid & 7) * 16;
+  }
+  i32x4_t ra[4], rb[4];
+  auto gload = [&](long off) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      ra[i] = *(const i32x4_t *)(pa[i] + off);
+      rb[i] = okb[i] ? *(const i32x4_t *)(pb[i] + off) : i32x4_t{0, 0, 0, 0};
+    }
+  };
+  auto lstore = [&](int buf) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int off = ((tid >> 3) + 32 * i) * GT_ROW + (tid & 7) * 16;
+      *(i32x4_t *)(&lds[buf][0][off]) = ra[i];
+      *(i32x4_t *)(&lds[buf][1][off]) = rb[i];
+    }
+  };
+
+  f32x16_t acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int v = 0; v < 16; v++) acc[a][b][v] = 0.f;
+
+  const int wm = wave >> 1, wn = wave & 1, r32 = lane & 31, h = lane >> 5;
+  gload(0);
+  lstore(0);
+  __syncthreads();
+  for (int s = 0; s < nsteps; s++) {
+    const int buf = s & 1;
+    if (s + 1 < nsteps) gload((long)(s + 1) * GT_KB);
+    const uint8_t *LA = lds[buf][0], *LB = lds[buf][1];
+    if (FMT == FMT_BF16) {
+#pragma unroll
+      for (int kk = 0; kk < 4; kk++) {
+        bf16x8_t fa[2], fb[2];
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+          fa[i] = *(const bf16x8_t *)(LA + (wm * 64 + i * 32 + r32) * GT_ROW + kk * 32 + h * 16);
+          fb[i] = *(const bf16x8_t *)(LB + (wn * 64 + i * 32 + r32) * GT_ROW + kk * 32 + h * 16);
+        }
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+          for (int b = 0; b < 2; b++)
+            acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[a], fb[b], acc[a][b], 0, 0, 0);
+      }
+    } else {
+#pragma unroll
+      for (int kk = 0; kk < 2; kk++) {
+        i32x8_t fa[2], fb[2];
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+          const uint8_t *qa = LA + (wm * 64 + i * 32 + r32) * GT_ROW + kk * 64 + h * 32;
+          const uint8_t *qb = LB + (wn * 64 + i * 32 + r32) * GT_ROW + kk * 64 + h * 32;
+          const i32x4_t a0 = *(const i32x4_t *)qa, a1 = *(const i32x4_t *)(qa + 16);
+          const i32x4_t b0 = *(const i32x4_t *)qb, b1 = *(const i32x4_t *)(qb + 16);
+          fa[i] = i32x8_t{a0[0], a0[1], a0[2], a0[3], a1[0], a1[1], a1[2], a1[3]};
+          fb[i] = i32x8_t{b0[0], b0[1], b0[2], b0[3], b1[0], b1[1], b1[2], b1[3]};
+        }
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+          for (int b = 0; b < 2; b++)
+            acc[a][b] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(fa[a], fb[b], acc[a][b], 0,
+                                                                        0, 0, 0, 0, 0);
+      }
+    }
+    if (s + 1 < nsteps) lstore(buf ^ 1);
+    __syncthreads();
+  }
+
+  float *C = g.C + (long)blockIdx.z * g.split_stride;
+#pragma unroll
+  for (int b = 0; b < 2; b++) {
+    const int c = n0 + wn * 64 + b * 32 + r32;
+    if (c >= ncols) continue;
+    const int phys = g.col_map ? g.col_map[c] : c;
+    const float cs = g.col_scale ? g.col_scale[phys] : 1.0f;
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int r = m0 + wm * 64 + a * 32 + q * 8 + h * 4;
+        float4 v = make_float4(acc[a][b][4 * q], acc[a][b][4 * q + 1], acc[a][b][4 * q + 2],
+                               acc[a][b][4 * q + 3]);
+        if (g.row_scale) {
+          v.x *= g.row_scale[r];
+          v.y *= g.row_scale[r + 1];
+          v.z *= g.row_scale[r + 2];
+          v.w *= g.row_scale[r + 3];
+        }
+        v.x *= cs;
+        v.y *= cs;
+        v.z *= cs;
+        v.w *= cs;
+        *(float4 *)(C + (long)phys * g.ldc + r) = v;
+      }
+  }
+}
+
+// Sum of the K-split slices in split order (deterministic), active columns.
+__global__ void gemm_split_sum_kernel(float *C, long ldc, long split_stride, int ksplit, int M,
+                                      int N, const int *ncols, const int *col_map) {
+  const int nc = ncols ? *ncols : N;
+  const long total = (long)nc * M;
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (long)gridDim.x * blockDim.x) {
+    const int c = (int)(e / M), r = (int)(e - (long)c * M);
+    const long o = (long)(col_map ? col_map[c] : c) * ldc + r;
+    float s = C[o];
+    for (int z = 1; z < ksplit; z++) s += C[o + z * split_stride];
+    C[o] = s;
+  }
+}
+
+// ------------------------------------------------------------ quantizers ----
+// float -> bf16, round to nearest even (finite inputs).
+__host__ __device__ __forceinline__ uint16_t f2bf16_rne(float f) {
+  uint32_t u;
+  __builtin_memcpy(&u, &f, 4);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+// float -> OCP e4m3 (round to nearest even in hardware); inputs are clamped
+// to +-448 first so nothing overflows.
+__device__ __forceinline__ uint32_t f2fp8x4(float a, float b, float c, float d) {
+  a = fminf(fmaxf(a, -FP8_MAX), FP8_MAX);
+  b = fminf(fmaxf(b, -FP8_MAX), FP8_MAX);
+  c = fminf(fmaxf(c, -FP8_MAX), FP8_MAX);
+  d = fminf(fmaxf(d, -FP8_MAX), FP8_MAX);
+  int w = __builtin_amdgcn_cvt_pk_fp8_f32(a, b, 0, false);
+  w = __builtin_amdgcn_cvt_pk_fp8_f32(c, d, w, true);
+  return (uint32_t)w;
+}
+
+}  // namespace xcs

@@ -1,0 +1,1358 @@
+// Many right-hand sides: one gen A (m x n) shared by `count` signals, solved
+// by CGIHT in lock step with the matrix products as MFMA GEMMs.
+//
+// The reference solves one signal per gaga_cs call (GAGA/gaga/gaga_gen.cu,
+// CGIHT_S_gen at GAGA/gaga/algorithms.cu:3689-3942); each of its matrix-vector
+// products becomes one column of a GEMM here:
+//   A x      -> A_rm * X      (A row-major copy; K = n)
+//   A^T r    -> A_cm^T * R    (A column-major; K = m)
+//   A p_S    -> A_rm * P
+// Every per-signal step between two products (residual, thresholds, step
+// sizes, the k-bin support update, the stopping rules) is the single-instance
+// engine's code run by one workgroup per signal.
+//
+// Precision: the products read A in BF16 (RNE) or OCP FP8 e4m3 with one scale
+// per column (s_j = max_i |A_ij| / 448) and the signal operand in the same
+// format (FP8: one scale per signal), accumulating in FP32; the vectors are
+// FP32 and the norms FP64.  The low-precision phase stops at its own floor
+// (tolerance c * ||y|| with c = 2^-7 for BF16, 2^-3 for FP8; FP8 continues in
+// BF16 from there).  Then, on the support found, a least-squares refinement:
+// the residual r = y - A_S x_S is formed from the FP32 A with FP64 sums, the
+// correction solves min ||r - A_S d|| by FP32 CG whose products are BF16
+// GEMMs, and x_S += d (mixed-precision iterative refinement).
+
+namespace xcs {
+
+constexpr int MR_THREADS = 256;
+constexpr int MR_KSPLIT = 4;  // K splits of the A * V products (K = n)
+
+struct MrhsDesc {
+  int m, n, k, count, num_bins, cstride, restart, maxiter, fmt;
+  float tol;
+  const float *y;
+  float *x, *grad, *thres, *gprev, *gprev_thres, *resid, *resid_update, *aux, *tol_lp;
+  int *bins, *bins2, *counters, *stage, *mode;
+  XState *st;
+  uint8_t *xb, *rb, *vb1, *vb2;
+  float *sx, *sr, *sv1, *sv2;
+  float *ax, *g, *w1, *w2;
+  const float *colscale;
+  int *active, *nactive, *ulist, *nu;
+  double *ref;  // per signal: gamma, |r|^2, |y|^2, spare
+  int *supp, *supp_cnt;
+  const float *A32;
+};
+
+__device__ __forceinline__ long mr_ldb(const MrhsDesc &D, int K) {
+  return (long)K * (D.fmt == FMT_BF16 ? 2 : 1);
+}
+
+// Per-signal descriptor for the engine's device functions (thread 0 fills it).
+__device__ void mr_view(XProblem &P, const MrhsDesc &D, int s, float tol) {
+  __builtin_memset(&P, 0, sizeof(P));
+  P.alg = ALG_CGIHT;
+  P.ens = ENS_GEN;
+  P.m = D.m;
+  P.n = D.n;
+  P.k = D.k;
+  P.lda = D.m;
+  P.num_bins = D.num_bins;
+  P.maxiter = D.maxiter;
+  P.restart = D.restart;
+  P.tol = tol;
+  const long n = D.n, m = D.m;
+  xset(P.y, D.y + s * m);
+  xset(P.x, D.x + s * n);
+  xset(P.thres, D.thres + s * n);
+  xset(P.grad, D.grad + s * n);
+  xset(P.gprev, D.gprev + s * n);
+  xset(P.gprev_thres, D.gprev_thres + s * n);
+  xset(P.bins, D.bins + s * n);
+  xset(P.bins2, D.bins2 + s * n);
+  xset(P.resid, D.resid + s * m);
+  xset(P.resid_update, D.resid_update + s * m);
+  xset(P.counters, D.counters + (long)s * D.cstride);
+  xset(P.st, D.st + s);
+}
+
+// v (times pre[i] when given) as column `col` of a B operand in the current
+// format; FP8 also writes the column scale.  v must be visible to the block.
+__device__ void mr_quant(Team &t, const MrhsDesc &D, const float *v, const float *pre, int len,
+                         uint8_t *base, float *scales, int col) {
+  if (D.fmt == FMT_BF16) {
+    uint16_t *o = (uint16_t *)(base + (long)col * len * 2);
+    for (int i = threadIdx.x; i < len; i += blockDim.x) {
+      const float a = pre ? v[i] * pre[i] : v[i];
+      o[i] = f2bf16_rne(a);
+    }
+    return;
+  }
+  float mx = 0.f;
+  for (int i = threadIdx.x; i < len; i += blockDim.x) mx = fmaxf(mx, fabsf(pre ? v[i] * pre[i] : v[i]));
+  double r[1] = {mx};
+  const int op[1] = {1};
+  team_reduce<double, 1>(t, r, op);
+  const float sc = r[0] > 0.0 ? (float)r[0] / FP8_MAX : 1.0f;
+  uint32_t *o = (uint32_t *)(base + (long)col * len);
+  for (int q = threadIdx.x; q < len / 4; q += blockDim.x) {
+    float a[4];
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      const int i = 4 * q + e;
+      a[e] = __fdiv_rn(pre ? v[i] * pre[i] : v[i], sc);
+    }
+    o[q] = f2fp8x4(a[0], a[1], a[2], a[3]);
+  }
+  if (threadIdx.x == 0) scales[col] = sc;
+}
+
+__device__ __forceinline__ const float *mr_pre(const MrhsDesc &D) {
+  return D.fmt == FMT_FP8 ? D.colscale : nullptr;
+}
+
+#define MR_PROLOGUE(tolv)                            \
+  __shared__ Lds L;                                  \
+  __shared__ XProblem sP;                            \
+  const int s = blockIdx.x;                          \
+  if (threadIdx.x == 0) mr_view(sP, D, s, (tolv));   \
+  __syncthreads();                                   \
+  Team t{0, 1, nullptr, 0, nullptr};                 \
+  const XProblem &P = sP;                            \
+  (void)L;                                           \
+  (void)t;
+
+// y as the first right-hand side operand (A^T y).
+__global__ void __launch_bounds__(MR_THREADS) mr_init0_kernel(MrhsDesc D) {
+  const int s = blockIdx.x;
+  Team t{0, 1, nullptr, 0, nullptr};
+  mr_quant(t, D, D.y + (long)s * D.m, nullptr, D.m, D.rb, D.sr, s);
+}
+
+// x = A^T y, first support (algorithms.cu:3752-3790 with solve_instance's init).
+__global__ void __launch_bounds__(MR_THREADS) mr_init1_kernel(MrhsDesc D, float c_lp) {
+  MR_PROLOGUE(D.tol);
+  XState S;
+  __builtin_memset(&S, 0, sizeof(S));
+  S.alpha_bin = 0.25f;
+  S.max_bin = (int)((float)P.num_bins * (1.0f - S.alpha_bin));
+  S.max_change = 1.0f;
+  S.diff = 1.f;
+  S.cycle = 1;
+  S.beta_to_zero = 1;
+  S.supp_change = 1;
+  for (int j = 0; j < 16; j++) S.evo[j] = 1.f;
+  const float *G = D.g + (long)s * D.n;
+  double e2 = 0.0;
+  for (int i = threadIdx.x; i < P.m; i += blockDim.x) {
+    const float r = P.y[i];
+    P.resid[i] = r;
+    e2 += (double)r * (double)r;
+  }
+  zero_counters(t, P);
+  float xm = 0.f;
+  for (int j = threadIdx.x; j < P.n; j += blockDim.x) {
+    const float v = G[j];
+    P.x[j] = v;
+    xm = fmaxf(xm, fabsf(v));
+  }
+  double v[2] = {e2, xm};
+  const int op[2] = {0, 1};
+  team_reduce<double, 2>(t, v, op);
+  S.err = (float)sqrt(v[0]);
+  S.err_start = S.err;
+  S.max_value = (float)v[1];
+  if (threadIdx.x == 0) {
+    D.tol_lp[s] = fmaxf(D.tol, c_lp * S.err);
+    D.ref[4 * s + 2] = v[0];
+  }
+  S.initialized = 1;
+  if (S.max_value == 0.f) {
+    S.done = 1;
+    if (threadIdx.x == 0) {
+      D.stage[s] = 1;
+      *P.st = S;
+    }
+    return;
+  }
+  S.slope = (float)(P.num_bins - 1) / S.max_value;
+  find_support_bin(t, L, P, P.x, P.bins, S, S.slope, S.max_value, &S.k_bin);
+  threshold_x(t, P, S.k_bin);
+  __syncthreads();
+  mr_quant(t, D, (const float *)P.x, mr_pre(D), P.n, D.xb, D.sx, s);
+  if (threadIdx.x == 0) {
+    D.stage[s] = 0;
+    *P.st = S;
+  }
+}
+
+// r^0 = y - A x (algorithms.cu:3791-3800).
+__global__ void __launch_bounds__(MR_THREADS) mr_init2_kernel(MrhsDesc D) {
+  MR_PROLOGUE(D.tol);
+  if (D.stage[s] != 0) return;
+  XState S = *P.st;
+  const float *AX = D.ax + (long)s * D.m;
+  double e2 = 0.0;
+  for (int i = threadIdx.x; i < P.m; i += blockDim.x) {
+    const float r = __fsub_rn(P.y[i], AX[i]);
+    P.resid[i] = r;
+    e2 += (double)r * (double)r;
+  }
+  double v[1] = {e2};
+  const int op[1] = {0};
+  team_reduce<double, 1>(t, v, op);
+  S.err = (float)sqrt(v[0]);
+  S.prev[15] = S.err;
+  mr_quant(t, D, (const float *)P.resid, nullptr, P.m, D.rb, D.sr, s);
+  if (threadIdx.x == 0) *P.st = S;
+}
+
+// first gradient and its restriction (algorithms.cu:3801-3808); loop entry test.
+__global__ void __launch_bounds__(MR_THREADS) mr_init3_kernel(MrhsDesc D) {
+  MR_PROLOGUE(D.tol_lp[blockIdx.x]);
+  if (D.stage[s] != 0) return;
+  XState S = *P.st;
+  const float *G = D.g + (long)s * D.n;
+  for (int j = threadIdx.x; j < P.n; j += blockDim.x) {
+    const float gj = G[j];
+    P.gprev[j] = gj;
+    P.gprev_thres[j] = (P.bins[j] <= S.k_bin) ? gj : 0.f;
+  }
+  for (int j = 0; j < 15; j++) S.evo[j] = S.evo[j + 1];
+  S.evo[15] = S.prev[14] - S.prev[15];
+  S.beta_to_zero = 1;
+  S.supp_change = 1;
+  int reason;
+  if (!should_continue(S, P, &reason)) {
+    S.done = 1;
+    S.stop_reason = reason;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (S.done) D.stage[s] = 1;
+    *P.st = S;
+  }
+}
+
+// Active signals (stage 0) and the ones taking an unrestricted step, in
+// ascending order; one workgroup.
+__global__ void __launch_bounds__(MR_THREADS) mr_list_kernel(MrhsDesc D) {
+  __shared__ int base_a, base_u;
+  if (threadIdx.x == 0) base_a = base_u = 0;
+  __syncthreads();
+  for (int c0 = 0; c0 < D.count; c0 += blockDim.x) {
+    const int s = c0 + threadIdx.x;
+    int act = 0, u = 0;
+    if (s < D.count && D.stage[s] == 0) {
+      act = 1;
+      u = (!D.restart && D.st[s].supp_change) ? 1 : 0;
+      D.mode[s] = u;
+    }
+    int ta, tu;
+    const int pa = wg_exclusive_scan(act, &ta);
+    const int pu = wg_exclusive_scan(u, &tu);
+    if (act) D.active[base_a + pa] = s;
+    if (u) D.ulist[base_u + pu] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      base_a += ta;
+      base_u += tu;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    *D.nactive = base_a;
+    *D.nu = base_u;
+  }
+}
+
+// Step part 1: r = y - A x (into resid for the restricted step, resid_update
+// for the unrestricted one), ||r|| into the history.
+__global__ void __launch_bounds__(MR_THREADS) mr_k1_kernel(MrhsDesc D) {
+  MR_PROLOGUE(D.tol_lp[blockIdx.x]);
+  if (D.stage[s] != 0) return;
+  XState S = *P.st;
+  const bool u = D.mode[s] != 0;
+  const float *AX = D.ax + (long)s * D.m;
+  float *r = u ? (float *)P.resid_update : (float *)P.resid;
+  double e2 = 0.0;
+  for (int i = threadIdx.x; i < P.m; i += blockDim.x) {
+    const float v = __fsub_rn(P.y[i], AX[i]);
+    r[i] = v;
+    e2 += (double)v * (double)v;
+  }
+  double v[1] = {e2};
+  const int op[1] = {0};
+  team_reduce<double, 1>(t, v, op);
+  push16(S.prev, (float)sqrt(v[0]));
+  mr_quant(t, D, r, nullptr, P.m, D.rb, D.sr, s);
+  if (threadIdx.x == 0) *P.st = S;
+}
+
+// Step part 2 (after g = A^T r): restricted: beta and the new direction,
+// whose restriction is the next product's operand (functions.cu:1004-1060);
+// unrestricted: the old restricted direction and the restricted gradient
+// are the operands (functions.cu:1107-1160).
+__global__ void __launch_bounds__(MR_THREADS) mr_k2_kernel(MrhsDesc D) {
+  MR_PROLOGUE(D.tol_lp[blockIdx.x]);
+  if (D.stage[s] != 0) return;
+  XState S = *P.st;
+  const int kb = S.k_bin;
+  const float *G = D.g + (long)s * D.n;
+  if (D.mode[s] == 0) {
+    float th2 = 0.f;
+    for (int j = threadIdx.x; j < P.n; j += blockDim.x) {
+      const float gj = G[j];
+      P.grad[j] = gj;
+      const float th = (P.bins[j] <= kb) ? gj : 0.f;
+      P.thres[j] = th;
+      th2 = fmaf(th, th, th2);
+    }
+    float beta;
+    {
+      double v[1] = {th2};
+      const int op[1] = {0};
+      team_reduce<double, 1>(t, v, op);
+      const float mu_new = (float)v[0];
+      if (S.beta_to_zero == 1) {
+        beta = 0.f;
+        S.mu = mu_new;
+      } else {
+        beta = S.mu;
+        S.mu = mu_new;
+        beta = (S.mu < 1000.0f * beta) ? S.mu / beta : 0.f;
+      }
+    }
+    float pmax = 0.f;
+    for (int j = threadIdx.x; j < P.n; j += blockDim.x) {
+      const float gp = __fadd_rn(__fmul_rn(beta, P.gprev[j]), P.grad[j]);
+      P.gprev[j] = gp;
+      P.gprev_thres[j] = (P.bins[j] <= kb) ? gp : 0.f;
+      pmax = fmaxf(pmax, fabsf(gp));
+    }
+    double v[1] = {pmax};
+    const int op[1] = {1};
+    team_reduce<double, 1>(t, v, op);
+    if (threadIdx.x == 0) D.aux[4 * s] = (float)v[0];
+    mr_quant(t, D, (const float *)P.gprev_thres, mr_pre(D), P.n, D.vb1, D.sv1, s);
+  } else {
+    for (int j = threadIdx.x; j < P.n; j += blockDim.x) {
+      const float gj = G[j];
+      P.grad[j] = gj;
+      P.thres[j] = (P.bins[j] <= kb) ? gj : 0.f;
+    }
+    __syncthreads();
+    mr_quant(t, D, (const float *)P.gprev_thres, mr_pre(D), P.n, D.vb1, D.sv1, s);
+    mr_quant(t, D, (const float *)P.thres, mr_pre(D), P.n, D.vb2, D.sv2, s);
+  }
+  if (threadIdx.x == 0) *P.st = S;
+}
+
+// Step part 3 (after the A p products): step length, x update, support
+// update and stopping rules (algorithms.cu:3835-3942).
+__global__ void __launch_bounds__(MR_THREADS) mr_k3_kernel(MrhsDesc D) {
+  MR_PROLOGUE(D.tol_lp[blockIdx.x]);
+  if (D.stage[s] != 0) return;
+  XState S = *P.st;
+  const int n = P.n, m = P.m;
+  const float *W1 = D.w1 + (long)s * m, *W2 = D.w2 + (long)s * m;
+  float xm = 0.f;
+  if (D.mode[s] == 0) {
+    float a2 = 0.f;
+    for (int i = threadIdx.x; i < m; i += blockDim.x) {
+      const float w = W1[i];
+      P.resid_update[i] = w;
+      a2 = fmaf(w, w, a2);
+    }
+    double v[1] = {a2};
+    const int op[1] = {0};
+    team_reduce<double, 1>(t, v, op);
+    float alpha = (float)v[0];
+    alpha = (S.mu < 1000.0f * alpha) ? S.mu / alpha : 0.f;
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+      const float xn = fmaf(alpha, P.gprev[j], P.x[j]);
+      P.x[j] = xn;
+      xm = fmaxf(xm, fabsf(xn));
+    }
+    S.max_change = 2.0f * alpha * D.aux[4 * s];
+  } else {
+    const int kb = S.k_bin;
+    float beta = 0.f;
+    if (S.beta_to_zero != 1) {
+      float num = 0.f, den = 0.f;
+      for (int i = threadIdx.x; i < m; i += blockDim.x) {
+        num = fmaf(W1[i], W2[i], num);
+        den = fmaf(W1[i], W1[i], den);
+      }
+      double v[2] = {num, den};
+      const int op[2] = {0, 0};
+      team_reduce<double, 2>(t, v, op);
+      beta = -((float)v[0]) / (float)v[1];
+    }
+    float pmax = 0.f, an = 0.f, mu2 = 0.f, ad = 0.f;
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+      const float gp = __fadd_rn(__fmul_rn(beta, P.gprev[j]), P.grad[j]);
+      P.gprev[j] = gp;
+      const float gpt = (P.bins[j] <= kb) ? gp : 0.f;
+      P.gprev_thres[j] = gpt;
+      pmax = fmaxf(pmax, fabsf(gp));
+      const float th = P.thres[j];
+      an = fmaf(gpt, th, an);
+      mu2 = fmaf(th, th, mu2);
+    }
+    for (int i = threadIdx.x; i < m; i += blockDim.x) {
+      const float r = __fadd_rn(__fmul_rn(beta, W1[i]), W2[i]);
+      P.resid[i] = r;
+      ad = fmaf(r, r, ad);
+    }
+    double v[4] = {an, ad, pmax, mu2};
+    const int op[4] = {0, 0, 1, 0};
+    team_reduce<double, 4>(t, v, op);
+    const float alpha = (float)v[0] / (float)v[1];
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+      const float xn = fmaf(alpha, P.gprev[j], P.x[j]);
+      P.x[j] = xn;
+      xm = fmaxf(xm, fabsf(xn));
+    }
+    S.mu = (float)v[3];
+    S.max_change = 2.0f * alpha * (float)v[2];
+  }
+  __syncthreads();
+  S.beta_to_zero = 0;
+  if (S.min_val <= S.max_change) {
+    const int kb_prev = S.k_bin;
+    rebin_x(t, L, P, S, xm, true);
+    float ch = 0.f;
+    for (int j = threadIdx.x; j < n; j += blockDim.x)
+      if (P.bins2[j] <= kb_prev && P.bins[j] > S.k_bin) ch = 1.f;
+    double v[1] = {ch};
+    const int op[1] = {1};
+    team_reduce<double, 1>(t, v, op);
+    S.supp_change = v[0] > 0.0 ? 1 : 0;
+    if (S.supp_change) {
+      if (P.restart) {
+        S.beta_to_zero = 1;
+      } else {
+        for (int j = threadIdx.x; j < n; j += blockDim.x)
+          P.gprev_thres[j] = (P.bins[j] <= S.k_bin) ? P.gprev[j] : 0.f;
+      }
+    }
+  } else {
+    S.supp_change = 0;
+    S.min_val = S.min_val - S.max_change;
+  }
+  threshold_x(t, P, S.k_bin);
+  __syncthreads();
+  S.err = S.prev[15];
+  stop_update(S, 749);
+  S.iter++;
+  int reason;
+  if (!should_continue(S, P, &reason)) {
+    S.done = 1;
+    S.stop_reason = reason;
+  } else {
+    mr_quant(t, D, (const float *)P.x, mr_pre(D), n, D.xb, D.sx, s);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (S.done) D.stage[s] = 1;
+    *P.st = S;
+  }
+}
+
+// FP8 phase finished: continue every signal in BF16 (D.fmt is BF16 here).
+__global__ void __launch_bounds__(MR_THREADS) mr_switch_kernel(MrhsDesc D, float c_lp) {
+  MR_PROLOGUE(D.tol);
+  XState S = *P.st;
+  if (S.max_value == 0.f) return;  // A^T y = 0: nothing to continue
+  S.done = 0;
+  S.diff = 1.f;
+  S.cycle = 1;
+  S.fail = 0;
+  for (int j = 0; j < 16; j++) S.evo[j] = 1.f;
+  mr_quant(t, D, (const float *)P.x, nullptr, P.n, D.xb, D.sx, s);
+  if (threadIdx.x == 0) {
+    D.tol_lp[s] = fmaxf(D.tol, c_lp * (float)sqrt(D.ref[4 * s + 2]));
+    D.stage[s] = 0;
+    *P.st = S;
+  }
+}
+
+// ------------------------------------------------------------ refinement ---
+// Support lists.  topk = 0: the nonzeros of x, for the active signals (the
+// exact-residual phase).  topk = 1: the refinement's support, the k largest
+// |x_j| (all nonzeros when there are at most k; ties at the k-th magnitude
+// taken in index order); x is cut to it and bins2 = 1 marks it.  The k-th
+// magnitude is found by a bitwise search on the IEEE bits of |x| (monotone
+// for non-negative floats).
+__global__ void __launch_bounds__(MR_THREADS) mr_ref_supp_kernel(MrhsDesc D, int topk) {
+  const int s = blockIdx.x;
+  if (!topk && D.stage[s] != 0) return;
+  const long n = D.n;
+  float *x = D.x + s * n;
+  int *mask = D.bins2 + s * n, *sup = D.supp + s * n;
+  __shared__ int base;
+  __shared__ int cnt_s[MR_THREADS / 64];
+  auto count_ge = [&](uint32_t thr) {
+    int c = 0;
+    for (int j = threadIdx.x; j < D.n; j += blockDim.x) {
+      const uint32_t u = __float_as_uint(x[j]) & 0x7FFFFFFFu;
+      c += (u >= thr) ? 1 : 0;
+    }
+    c = wave_sum_i(c);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) cnt_s[threadIdx.x >> 6] = c;
+    __syncthreads();
+    int t = 0;
+    for (int w = 0; w < (int)blockDim.x / 64; w++) t += cnt_s[w];
+    return t;
+  };
+  uint32_t thr = 0u;
+  int quota = 0;  // entries equal to thr that may still be taken
+  const bool all_nz = !(topk && count_ge(1u) > D.k);
+  if (!all_nz) {
+    uint32_t t = 0;
+    for (int bit = 30; bit >= 0; bit--) {
+      const uint32_t cand = t | (1u << bit);
+      if (count_ge(cand) >= D.k) t = cand;
+    }
+    thr = t;
+    quota = D.k - count_ge(t + 1u);
+  }
+  if (threadIdx.x == 0) base = 0;
+  __syncthreads();
+  int eq_taken = 0;
+  for (int j0 = 0; j0 < D.n; j0 += blockDim.x) {
+    const int j = j0 + threadIdx.x;
+    const uint32_t u = j < D.n ? (__float_as_uint(x[j]) & 0x7FFFFFFFu) : 0u;
+    const int eq = (!all_nz && j < D.n && u == thr) ? 1 : 0;
+    int eq_tot;
+    const int eq_pos = wg_exclusive_scan(eq, &eq_tot);
+    const int in = j < D.n && (all_nz ? (u != 0u) : ((u > thr) || (eq && eq_taken + eq_pos < quota)));
+    eq_taken += eq_tot;
+    if (j < D.n && topk) {
+      mask[j] = in;
+      if (!in) x[j] = 0.f;
+    }
+    int tot;
+    const int pos = wg_exclusive_scan(in, &tot);
+    if (in) sup[base + pos] = j;
+    __syncthreads();
+    if (threadIdx.x == 0) base += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) D.supp_cnt[s] = base;
+}
+
+// r = y - A_S x_S from the FP32 A with FP64 sums over the support list of
+// each selected signal.  Pass 1: one workgroup per (256-row block, signal,
+// slice z of MR_SLICES), one row per thread, over the support entries
+// c = z*256 .. (mod MR_SLICES*256) staged through LDS; FP64 row sums into
+// rowpart.  Pass 2 (mr_resid_sum_kernel): slices summed in order, r and the
+// block's FP64 partial of ||r||^2 (part[s * nblk + block]).
+// which: 0 every signal, 1 the active ones (stage 0; r into resid or, for an
+// unrestricted step, resid_update).
+constexpr int MR_SLICES = 8;
+__global__ void __launch_bounds__(MR_THREADS) mr_exact_resid_kernel(MrhsDesc D, int which,
+                                                                    double *rowpart) {
+  __shared__ int sj[MR_THREADS];
+  __shared__ float sv[MR_THREADS];
+  const int s = blockIdx.y, z = blockIdx.z;
+  if (which == 1 && D.stage[s] != 0) return;
+  const long n = D.n, m = D.m;
+  const float *x = D.x + s * n;
+  const int *sup = D.supp + s * n;
+  const int cnt = D.supp_cnt[s];
+  const int i = blockIdx.x * MR_THREADS + threadIdx.x;
+  double acc = 0.0;
+  for (int c0 = z * MR_THREADS; c0 < cnt; c0 += MR_SLICES * MR_THREADS) {
+    __syncthreads();
+    if (c0 + (int)threadIdx.x < cnt) {
+      const int j = sup[c0 + threadIdx.x];
+      sj[threadIdx.x] = j;
+      sv[threadIdx.x] = x[j];
+    }
+    __syncthreads();
+    const int c1 = min(MR_THREADS, cnt - c0);
+    if (i < D.m) {
+      int c = 0;
+      for (; c + 4 <= c1; c += 4) {
+        const float a0 = D.A32[(long)sj[c] * m + i], a1 = D.A32[(long)sj[c + 1] * m + i];
+        const float a2 = D.A32[(long)sj[c + 2] * m + i], a3 = D.A32[(long)sj[c + 3] * m + i];
+        acc = fma((double)a0, (double)sv[c], acc);
+        acc = fma((double)a1, (double)sv[c + 1], acc);
+        acc = fma((double)a2, (double)sv[c + 2], acc);
+        acc = fma((double)a3, (double)sv[c + 3], acc);
+      }
+      for (; c < c1; c++) acc = fma((double)D.A32[(long)sj[c] * m + i], (double)sv[c], acc);
+    }
+  }
+  if (i < D.m) rowpart[((long)s * MR_SLICES + z) * m + i] = acc;
+}
+
+__global__ void __launch_bounds__(MR_THREADS) mr_resid_sum_kernel(MrhsDesc D, int which,
+                                                                  const double *rowpart,
+                                                                  double *part) {
+  __shared__ double red[MR_THREADS / 64];
+  const int s = blockIdx.y;
+  if (which == 1 && D.stage[s] != 0) return;
+  const long m = D.m;
+  const int i = blockIdx.x * MR_THREADS + threadIdx.x;
+  double e2 = 0.0;
+  if (i < D.m) {
+    double a = 0.0;
+    for (int z = 0; z < MR_SLICES; z++) a += rowpart[((long)s * MR_SLICES + z) * m + i];
+    const double ri = (double)D.y[s * m + i] - a;
+    float *r = (which == 1 && D.mode[s]) ? D.resid_update : D.resid;
+    r[s * m + i] = (float)ri;
+    e2 = ri * ri;
+  }
+  e2 = wave_sum(e2);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = e2;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < MR_THREADS / 64; w++) t += red[w];
+    part[(long)s * gridDim.x + blockIdx.x] = t;
+  }
+}
+
+__device__ __forceinline__ double mr_part_sum(const double *part, int s, int nblk) {
+  double t = 0.0;
+  for (int b = 0; b < nblk; b++) t += part[(long)s * nblk + b];
+  return t;
+}
+
+// Refinement residual (after mr_exact_resid_kernel, which 0): ||r||^2 into
+// ref; d = 0; r as the next A^T operand (BF16).
+__global__ void __launch_bounds__(MR_THREADS) mr_ref_resid_kernel(MrhsDesc D, const double *part,
+                                                                  int nblk) {
+  const int s = blockIdx.x;
+  const long n = D.n, m = D.m;
+  float *d = D.thres + s * n;
+  for (int j = threadIdx.x; j < D.n; j += blockDim.x) d[j] = 0.f;
+  if (threadIdx.x == 0) D.ref[4 * s + 1] = mr_part_sum(part, s, nblk);
+  __syncthreads();
+  Team t{0, 1, nullptr, 0, nullptr};
+  mr_quant(t, D, D.resid + s * m, nullptr, D.m, D.rb, D.sr, s);
+}
+
+// Step part 1 with the exact residual (after mr_exact_resid_kernel, which 1):
+// ||r|| into the history, r as the next A^T operand.
+__global__ void __launch_bounds__(MR_THREADS) mr_k1x_kernel(MrhsDesc D, const double *part,
+                                                            int nblk) {
+  MR_PROLOGUE(D.tol_lp[blockIdx.x]);
+  if (D.stage[s] != 0) return;
+  XState S = *P.st;
+  const float *r = D.mode[s] ? (const float *)P.resid_update : (const float *)P.resid;
+  push16(S.prev, (float)sqrt(mr_part_sum(part, s, nblk)));
+  mr_quant(t, D, r, nullptr, P.m, D.rb, D.sr, s);
+  if (threadIdx.x == 0) *P.st = S;
+}
+
+// After a refinement: a signal whose residual exceeds tol (the reference's
+// stopping test, now with the exact residual) resumes CGIHT from its refined
+// x with the exact residual, a fresh stopping history and a restarted
+// direction; the others stay finished.
+__global__ void __launch_bounds__(MR_THREADS) mr_mark_kernel(MrhsDesc D) {
+  const int s = blockIdx.x;
+  if (threadIdx.x != 0) return;
+  XState S = D.st[s];
+  const double rn = sqrt(D.ref[4 * s + 1]);
+  if (S.max_value == 0.f || !(rn > (double)D.tol)) {
+    D.stage[s] = 1;
+    return;
+  }
+  S.done = 0;
+  S.stop_reason = 0;
+  S.diff = 1.f;
+  S.cycle = 1;
+  S.fail = 0;
+  S.beta_to_zero = 1;
+  S.supp_change = 1;
+  for (int j = 0; j < 16; j++) S.evo[j] = 1.f;
+  D.st[s] = S;
+  D.tol_lp[s] = D.tol;
+  D.stage[s] = 0;
+}
+
+// CG on A_S^T A_S d = A_S^T r, part 1 (after s = A^T r): gamma = ||s_S||^2,
+// p = s_S + beta p (beta = 0 on the first step); p as the next A operand.
+__global__ void __launch_bounds__(MR_THREADS) mr_ref_dir_kernel(MrhsDesc D, int first) {
+  const int s = blockIdx.x;
+  const long n = D.n;
+  const float *G = D.g + s * n;
+  const int *mask = D.bins2 + s * n;
+  float *p = D.gprev_thres + s * n, *sg = D.grad + s * n;
+  Team t{0, 1, nullptr, 0, nullptr};
+  double g2 = 0.0;
+  for (int j = threadIdx.x; j < D.n; j += blockDim.x) {
+    const float v = mask[j] ? G[j] : 0.f;
+    sg[j] = v;
+    g2 += (double)v * (double)v;
+  }
+  double v[1] = {g2};
+  const int op[1] = {0};
+  team_reduce<double, 1>(t, v, op);
+  const double gold = D.ref[4 * s];
+  const float beta = (first || gold == 0.0) ? 0.f : (float)(v[0] / gold);
+  for (int j = threadIdx.x; j < D.n; j += blockDim.x) p[j] = fmaf(beta, first ? 0.f : p[j], sg[j]);
+  __syncthreads();
+  if (threadIdx.x == 0) D.ref[4 * s] = v[0];
+  mr_quant(t, D, p, nullptr, D.n, D.vb1, D.sv1, s);
+}
+
+// CG part 2 (after q = A p): alpha = gamma / ||q||^2, d += alpha p,
+// r -= alpha q; r as the next A^T operand.
+__global__ void __launch_bounds__(MR_THREADS) mr_ref_step_kernel(MrhsDesc D) {
+  const int s = blockIdx.x;
+  const long n = D.n, m = D.m;
+  const float *q = D.w1 + s * m, *p = D.gprev_thres + s * n;
+  float *r = D.resid + s * m, *d = D.thres + s * n;
+  Team t{0, 1, nullptr, 0, nullptr};
+  double q2 = 0.0;
+  for (int i = threadIdx.x; i < D.m; i += blockDim.x) q2 += (double)q[i] * (double)q[i];
+  double v[1] = {q2};
+  const int op[1] = {0};
+  team_reduce<double, 1>(t, v, op);
+  const double gam = D.ref[4 * s];
+  const float alpha = v[0] > 0.0 ? (float)(gam / v[0]) : 0.f;
+  for (int j = threadIdx.x; j < D.n; j += blockDim.x) d[j] = fmaf(alpha, p[j], d[j]);
+  for (int i = threadIdx.x; i < D.m; i += blockDim.x) r[i] = fmaf(-alpha, q[i], r[i]);
+  __syncthreads();
+  mr_quant(t, D, r, nullptr, D.m, D.rb, D.sr, s);
+}
+
+// x_S += d.
+__global__ void __launch_bounds__(MR_THREADS) mr_ref_update_kernel(MrhsDesc D) {
+  const int s = blockIdx.x;
+  const long n = D.n;
+  float *x = D.x + s * n;
+  const float *d = D.thres + s * n;
+  const int *mask = D.bins2 + s * n;
+  for (int j = threadIdx.x; j < D.n; j += blockDim.x)
+    if (mask[j]) x[j] = __fadd_rn(x[j], d[j]);
+}
+
+// y = A x_true for every signal from the FP32 A, FP64 sums.
+__global__ void __launch_bounds__(MR_THREADS) mr_make_y_kernel(const float *A, int m,
+                                                               const float *xt, const int *sp,
+                                                               int k, float *Y) {
+  const int s = blockIdx.x;
+  const int *sup = sp + (long)s * k;
+  const float *x = xt + (long)s * k;  // values in support order
+  for (int i = threadIdx.x; i < m; i += blockDim.x) {
+    double a = 0.0;
+    for (int c = 0; c < k; c++) a = fma((double)A[(long)sup[c] * m + i], (double)x[c], a);
+    Y[(long)s * m + i] = (float)a;
+  }
+}
+
+// --------------------------------------------------------- A quantization --
+// Column scales s_j = max_i |A_ij| / 448 (one wave per column).
+__global__ void mr_colscale_kernel(const float *A, int m, int n, float *cs) {
+  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
+  if (wave >= n) return;
+  float mx = 0.f;
+  for (int i = lane; i < m; i += 64) mx = fmaxf(mx, fabsf(A[(long)wave * m + i]));
+  mx = wave_max(mx);
+  if (lane == 0) cs[wave] = mx > 0.f ? mx / FP8_MAX : 1.0f;
+}
+
+// Quantized copies of A (m x n, column-major, lda = m) in both layouts, from
+// 64 x 64 tiles: cm[j*m + i] and rm[i*n + j] hold q(A_ij) (BF16: RNE(A_ij);
+// FP8: e4m3(A_ij / s_j)).
+template <int FMT>
+__global__ void __launch_bounds__(256) mr_quant_a_kernel(const float *A, int m, int n,
+                                                          const float *cs, uint8_t *cm,
+                                                          uint8_t *rm) {
+  __shared__ float tile[64][65];
+  const int i0 = blockIdx.x * 64, j0 = blockIdx.y * 64;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  for (int jj = ty; jj < 64; jj += 4) {
+    float v = A[(long)(j0 + jj) * m + i0 + tx];
+    if (FMT == FMT_FP8) v = __fdiv_rn(v, cs[j0 + jj]);
+    tile[jj][tx] = v;
+  }
+  __syncthreads();
+  if (FMT == FMT_BF16) {
+    for (int jj = ty; jj < 64; jj += 4)
+      ((uint16_t *)cm)[(long)(j0 + jj) * m + i0 + tx] = f2bf16_rne(tile[jj][tx]);
+    for (int ii = ty; ii < 64; ii += 4)
+      ((uint16_t *)rm)[(long)(i0 + ii) * n + j0 + tx] = f2bf16_rne(tile[tx][ii]);
+  } else {
+    // 16 threads x 4 values per 64-wide row
+    const int q = threadIdx.x & 15, rr = threadIdx.x >> 4;
+    for (int jj = rr; jj < 64; jj += 16)
+      ((uint32_t *)(cm + (long)(j0 + jj) * m + i0))[q] =
+          f2fp8x4(tile[jj][4 * q], tile[jj][4 * q + 1], tile[jj][4 * q + 2], tile[jj][4 * q + 3]);
+    for (int ii = rr; ii < 64; ii += 16)
+      ((uint32_t *)(rm + (long)(i0 + ii) * n + j0))[q] =
+          f2fp8x4(tile[4 * q][ii], tile[4 * q + 1][ii], tile[4 * q + 2][ii], tile[4 * q + 3][ii]);
+  }
+}
+
+}  // namespace xcs
+
+// ================================================================== host ====
+// (included by xcs_api.hip after its error and generator helpers)
+
+namespace xcs {
+
+// Dequantize (for the kernel-level GEMM check): one byte or bf16 per element.
+__global__ void mr_decode_kernel(const uint8_t *q, int fmt, long cnt, int per_row,
+                                 const float *row_scale, float *out) {
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < cnt;
+       e += (long)gridDim.x * blockDim.x) {
+    float v;
+    if (fmt == FMT_BF16) {
+      const uint32_t u = (uint32_t)((const uint16_t *)q)[e] << 16;
+      __builtin_memcpy(&v, &u, 4);
+    } else {
+      v = __builtin_amdgcn_cvt_f32_fp8((int)q[e], 0) * row_scale[e / per_row];
+    }
+    out[e] = v;
+  }
+}
+
+// Row-wise quantization of a row-major matrix (rows x len): BF16 RNE, or FP8
+// of v / scale[row].
+__global__ void mr_quant_rows_kernel(const float *v, int rows, int len, int fmt,
+                                     const float *scale, uint8_t *out) {
+  const long cnt4 = (long)rows * len / 4;
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < cnt4;
+       e += (long)gridDim.x * blockDim.x) {
+    const long i = 4 * e;
+    if (fmt == FMT_BF16) {
+      for (int u = 0; u < 4; u++) ((uint16_t *)out)[i + u] = f2bf16_rne(v[i + u]);
+    } else {
+      const float s = scale[i / len];
+      ((uint32_t *)out)[e] = f2fp8x4(__fdiv_rn(v[i], s), __fdiv_rn(v[i + 1], s),
+                                     __fdiv_rn(v[i + 2], s), __fdiv_rn(v[i + 3], s));
+    }
+  }
+}
+
+}  // namespace xcs
+
+struct xcs_mrhs {
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  int m = 0, n = 0, k = 0, count = 0, prec = FMT_BF16;
+  int ref_outer = 3, ref_inner = 5;
+  float c16 = 0.0078125f, c8 = 0.125f;
+  MrhsDesc D{};
+  float *A32 = nullptr, *cs = nullptr;
+  uint8_t *a16cm = nullptr, *a16rm = nullptr, *a8cm = nullptr, *a8rm = nullptr;
+  std::vector<void *> allocs;
+  std::vector<std::vector<float>> x_true;
+  int lp_iters[3] = {0, 0, 0};
+  float last_ms = 0.f, gen_ms = 0.f;
+  int *h_nactive = nullptr;  // pinned
+  double *part = nullptr;    // exact-residual partials (count x ceil(m / 256))
+  double *rowpart = nullptr; // exact-residual row sums per slice (count x MR_SLICES x m)
+  int nblk = 1;
+  int polish_max = 16;
+  xcs_options opts{};
+  float phase_ms[6] = {0, 0, 0, 0, 0, 0};  // init, low-precision, bf16 continuation, refine, exact phase, refine
+  ~xcs_mrhs() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (void *q : allocs) (void)hipFree(q);
+    if (h_nactive) (void)hipHostFree(h_nactive);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+  template <class T>
+  int alloc(T **q, size_t cnt) {
+    void *r = nullptr;
+    const size_t bytes = std::max<size_t>(cnt, 1) * sizeof(T);
+    hipError_t e = hipMalloc(&r, bytes);
+    if (e != hipSuccess)
+      return set_err(XCS_ENOMEM, "hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(e));
+    allocs.push_back(r);
+    e = hipMemsetAsync(r, 0, bytes, stream);
+    if (e != hipSuccess) return set_err(XCS_EHIP, "hipMemsetAsync: %s", hipGetErrorString(e));
+    *q = (T *)r;
+    return XCS_OK;
+  }
+};
+
+// One product for the active signals: transpose: G = A^T R (M = n, K = m);
+// else C = A V (M = m, K = n, split over K then summed in split order).
+static int mr_gemm(xcs_mrhs *h, int fmt, bool transpose, const uint8_t *B, float *C,
+                   const float *bscale, const int *ncols, const int *map) {
+  const int esz = fmt == FMT_BF16 ? 2 : 1;
+  GemmArgs g{};
+  if (transpose) {
+    g.A = fmt == FMT_BF16 ? h->a16cm : h->a8cm;
+    g.M = h->n;
+    g.K = h->m;
+    g.ldc = h->n;
+    g.ksplit = 1;
+    g.row_scale = fmt == FMT_FP8 ? h->cs : nullptr;
+  } else {
+    g.A = fmt == FMT_BF16 ? h->a16rm : h->a8rm;
+    g.M = h->m;
+    g.K = h->n;
+    g.ldc = h->m;
+    g.ksplit = MR_KSPLIT;
+    g.split_stride = (long)h->count * h->m;
+  }
+  g.lda_bytes = (long)g.K * esz;
+  g.B = B;
+  g.ldb_bytes = (long)g.K * esz;
+  g.C = C;
+  g.N = h->count;
+  g.ncols = ncols;
+  g.col_map = map;
+  g.col_scale = fmt == FMT_FP8 ? bscale : nullptr;
+  const dim3 grid(g.M / GT_M, (h->count + GT_N - 1) / GT_N, g.ksplit);
+  if (fmt == FMT_BF16)
+    hipLaunchKernelGGL(gemm_tn_kernel<FMT_BF16>, grid, dim3(GT_THREADS), 0, h->stream, g);
+  else
+    hipLaunchKernelGGL(gemm_tn_kernel<FMT_FP8>, grid, dim3(GT_THREADS), 0, h->stream, g);
+  HIP_TRY(hipGetLastError());
+  if (g.ksplit > 1) {
+    hipLaunchKernelGGL(gemm_split_sum_kernel, dim3(2048), dim3(256), 0, h->stream, C, g.ldc,
+                       g.split_stride, g.ksplit, g.M, h->count, ncols, map);
+    HIP_TRY(hipGetLastError());
+  }
+  return XCS_OK;
+}
+
+static int mr_create(xcs_mrhs *h, int m, int n, int k, int count, const char *precision,
+                     const xcs_options *o) {
+  if (!precision || (strcmp(precision, "bf16") && strcmp(precision, "fp8")))
+    return set_err(XCS_EINVAL, "precision must be bf16 or fp8");
+  if (m <= 0 || n <= 0 || count <= 0 || k <= 0 || k > m)
+    return set_err(XCS_EINVAL, "need 0 < k <= m, n > 0, count > 0");
+  if (m % 128 || n % 512)
+    return set_err(XCS_EINVAL, "many-rhs products need m %% 128 == 0 and n %% 512 == 0");
+  XCS_TRY(use_device(o->gpu_number));
+  HIP_TRY(hipGetDevice(&h->dev));
+  HIP_TRY(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  HIP_TRY(hipHostMalloc((void **)&h->h_nactive, 64, hipHostMallocDefault));
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  HIP_TRY(hipEventRecord(e0, h->stream));
+  h->m = m;
+  h->n = n;
+  h->k = k;
+  h->count = count;
+  h->prec = strcmp(precision, "fp8") ? FMT_BF16 : FMT_FP8;
+  h->opts = *o;
+  const size_t S = (size_t)count, mn = (size_t)m * n;
+  // A in FP32 (generator key (seed, instance)), BF16 in both layouts, FP8 too
+  XCS_TRY(h->alloc(&h->A32, mn));
+  hipLaunchKernelGGL(gen_matrix_kernel, dim3((int)std::min<size_t>(65536, (mn + 255) / 256)),
+                     dim3(256), 0, h->stream, h->A32, m, n, m, o->seed, o->instance);
+  HIP_TRY(hipGetLastError());
+  XCS_TRY(h->alloc(&h->cs, (size_t)n));
+  XCS_TRY(h->alloc(&h->a16cm, 2 * mn));
+  XCS_TRY(h->alloc(&h->a16rm, 2 * mn));
+  hipLaunchKernelGGL(mr_quant_a_kernel<FMT_BF16>, dim3(m / 64, n / 64), dim3(256), 0, h->stream,
+                     h->A32, m, n, h->cs, h->a16cm, h->a16rm);
+  HIP_TRY(hipGetLastError());
+  if (h->prec == FMT_FP8) {
+    XCS_TRY(h->alloc(&h->a8cm, mn));
+    XCS_TRY(h->alloc(&h->a8rm, mn));
+    hipLaunchKernelGGL(mr_colscale_kernel, dim3((n * 64 + 255) / 256), dim3(256), 0, h->stream,
+                       h->A32, m, n, h->cs);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(mr_quant_a_kernel<FMT_FP8>, dim3(m / 64, n / 64), dim3(256), 0, h->stream,
+                       h->A32, m, n, h->cs, h->a8cm, h->a8rm);
+    HIP_TRY(hipGetLastError());
+  }
+  // signals: key (seed, instance + 1 + s)
+  MrhsDesc &D = h->D;
+  D.m = m;
+  D.n = n;
+  D.k = k;
+  D.count = count;
+  D.num_bins = o->num_bins > 0 ? o->num_bins : std::max(n / 20, 1000);
+  D.cstride = round_up(D.num_bins + 64, 64);
+  D.restart = o->restart;
+  D.maxiter = o->maxiter > 0 ? o->maxiter : 5000;
+  D.tol = o->tol_reference_default ? (float)(10 ^ (-4)) : o->tol;
+  D.A32 = h->A32;
+  D.colscale = h->cs;
+  float *y;
+  XCS_TRY(h->alloc(&y, S * m));
+  D.y = y;
+  XCS_TRY(h->alloc(&D.x, S * n));
+  XCS_TRY(h->alloc(&D.grad, S * n));
+  XCS_TRY(h->alloc(&D.thres, S * n));
+  XCS_TRY(h->alloc(&D.gprev, S * n));
+  XCS_TRY(h->alloc(&D.gprev_thres, S * n));
+  XCS_TRY(h->alloc(&D.resid, S * m));
+  XCS_TRY(h->alloc(&D.resid_update, S * m));
+  XCS_TRY(h->alloc(&D.aux, 4 * S));
+  XCS_TRY(h->alloc(&D.tol_lp, S));
+  XCS_TRY(h->alloc(&D.bins, S * n));
+  XCS_TRY(h->alloc(&D.bins2, S * n));
+  XCS_TRY(h->alloc(&D.counters, S * D.cstride));
+  XCS_TRY(h->alloc(&D.stage, S));
+  XCS_TRY(h->alloc(&D.mode, S));
+  XCS_TRY(h->alloc(&D.st, S));
+  XCS_TRY(h->alloc(&D.xb, 2 * S * n));
+  XCS_TRY(h->alloc(&D.vb1, 2 * S * n));
+  XCS_TRY(h->alloc(&D.vb2, 2 * S * n));
+  XCS_TRY(h->alloc(&D.rb, 2 * S * m));
+  XCS_TRY(h->alloc(&D.sx, S));
+  XCS_TRY(h->alloc(&D.sr, S));
+  XCS_TRY(h->alloc(&D.sv1, S));
+  XCS_TRY(h->alloc(&D.sv2, S));
+  XCS_TRY(h->alloc(&D.ax, MR_KSPLIT * S * m));
+  XCS_TRY(h->alloc(&D.w1, MR_KSPLIT * S * m));
+  XCS_TRY(h->alloc(&D.w2, MR_KSPLIT * S * m));
+  XCS_TRY(h->alloc(&D.g, S * n));
+  XCS_TRY(h->alloc(&D.active, S));
+  XCS_TRY(h->alloc(&D.ulist, S));
+  XCS_TRY(h->alloc(&D.nactive, 64));
+  XCS_TRY(h->alloc(&D.nu, 64));
+  XCS_TRY(h->alloc(&D.ref, 4 * S));
+  XCS_TRY(h->alloc(&D.supp, S * n));
+  XCS_TRY(h->alloc(&D.supp_cnt, S));
+  h->nblk = (m + MR_THREADS - 1) / MR_THREADS;
+  XCS_TRY(h->alloc(&h->part, S * h->nblk));
+  XCS_TRY(h->alloc(&h->rowpart, S * MR_SLICES * m));
+  // planted signals and y = A x_true (FP32 A, FP64 sums)
+  h->x_true.resize(S);
+  std::vector<float> vals(S * k);
+  std::vector<int> sup_all(S * k);
+  for (int s = 0; s < count; s++) {
+    std::vector<int> sup;
+    host_sparse_vector(n, k, o->vec_distribution, o->seed, o->instance + 1u + (uint32_t)s,
+                       h->x_true[s], sup);
+    if ((int)sup.size() != k) return set_err(XCS_EINVAL, "internal: support size");
+    for (int c = 0; c < k; c++) {
+      sup_all[(size_t)s * k + c] = sup[c];
+      vals[(size_t)s * k + c] = h->x_true[s][sup[c]];
+    }
+  }
+  float *d_vals;
+  int *d_sup;
+  XCS_TRY(h->alloc(&d_vals, S * k));
+  XCS_TRY(h->alloc(&d_sup, S * k));
+  HIP_TRY(hipMemcpyAsync(d_vals, vals.data(), 4 * vals.size(), hipMemcpyHostToDevice, h->stream));
+  HIP_TRY(hipMemcpyAsync(d_sup, sup_all.data(), 4 * sup_all.size(), hipMemcpyHostToDevice, h->stream));
+  hipLaunchKernelGGL(mr_make_y_kernel, dim3(count), dim3(MR_THREADS), 0, h->stream, h->A32, m,
+                     d_vals, d_sup, k, y);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(e1, h->stream));
+  HIP_TRY(hipEventSynchronize(e1));
+  HIP_TRY(hipEventElapsedTime(&h->gen_ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return XCS_OK;
+}
+
+static int mr_exact_resid(xcs_mrhs *h, int which) {
+  hipLaunchKernelGGL(mr_exact_resid_kernel, dim3(h->nblk, h->count, MR_SLICES), dim3(MR_THREADS), 0,
+                     h->stream, h->D, which, h->rowpart);
+  hipLaunchKernelGGL(mr_resid_sum_kernel, dim3(h->nblk, h->count), dim3(MR_THREADS), 0, h->stream,
+                     h->D, which, h->rowpart, h->part);
+  HIP_TRY(hipGetLastError());
+  return XCS_OK;
+}
+
+// One lock-step CGIHT iteration over the active signals (format D.fmt).
+static int mr_iteration(xcs_mrhs *h, bool exact) {
+  MrhsDesc &D = h->D;
+  const int f = D.fmt;
+  const dim3 gs(h->count), bs(MR_THREADS);
+  hipLaunchKernelGGL(mr_list_kernel, dim3(1), bs, 0, h->stream, D);
+  if (exact) {
+    hipLaunchKernelGGL(mr_ref_supp_kernel, gs, bs, 0, h->stream, D, 0);
+    XCS_TRY(mr_exact_resid(h, 1));
+    hipLaunchKernelGGL(mr_k1x_kernel, gs, bs, 0, h->stream, D, h->part, h->nblk);
+  } else {
+    XCS_TRY(mr_gemm(h, f, false, D.xb, D.ax, D.sx, D.nactive, D.active));
+    hipLaunchKernelGGL(mr_k1_kernel, gs, bs, 0, h->stream, D);
+  }
+  XCS_TRY(mr_gemm(h, f, true, D.rb, D.g, D.sr, D.nactive, D.active));
+  hipLaunchKernelGGL(mr_k2_kernel, gs, bs, 0, h->stream, D);
+  XCS_TRY(mr_gemm(h, f, false, D.vb1, D.w1, D.sv1, D.nactive, D.active));
+  XCS_TRY(mr_gemm(h, f, false, D.vb2, D.w2, D.sv2, D.nu, D.ulist));
+  hipLaunchKernelGGL(mr_k3_kernel, gs, bs, 0, h->stream, D);
+  HIP_TRY(hipGetLastError());
+  return XCS_OK;
+}
+
+// Iterate until no signal is active (checked every 4 iterations) or the
+// iteration cap; returns the iterations issued.
+static int mr_phase(xcs_mrhs *h, bool exact, int *issued) {
+  MrhsDesc &D = h->D;
+  int it = 0;
+  const int cap = exact ? h->polish_max : D.maxiter + 2;
+  while (it < cap) {
+    hipLaunchKernelGGL(mr_list_kernel, dim3(1), dim3(MR_THREADS), 0, h->stream, D);
+    HIP_TRY(hipMemcpyAsync(h->h_nactive, D.nactive, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    if (*h->h_nactive == 0) break;
+    for (int q = 0; q < 4 && it < cap; q++, it++) XCS_TRY(mr_iteration(h, exact));
+  }
+  *issued = it;
+  return XCS_OK;
+}
+
+static int mr_refine(xcs_mrhs *h) {
+  MrhsDesc &D = h->D;
+  D.fmt = FMT_BF16;
+  const dim3 gs(h->count), bs(MR_THREADS);
+  hipLaunchKernelGGL(mr_ref_supp_kernel, gs, bs, 0, h->stream, D, 1);
+  for (int o = 0; o < h->ref_outer; o++) {
+    XCS_TRY(mr_exact_resid(h, 0));
+    hipLaunchKernelGGL(mr_ref_resid_kernel, gs, bs, 0, h->stream, D, h->part, h->nblk);
+    XCS_TRY(mr_gemm(h, FMT_BF16, true, D.rb, D.g, nullptr, nullptr, nullptr));
+    hipLaunchKernelGGL(mr_ref_dir_kernel, gs, bs, 0, h->stream, D, 1);
+    for (int j = 0; j < h->ref_inner; j++) {
+      XCS_TRY(mr_gemm(h, FMT_BF16, false, D.vb1, D.w1, nullptr, nullptr, nullptr));
+      hipLaunchKernelGGL(mr_ref_step_kernel, gs, bs, 0, h->stream, D);
+      if (j + 1 == h->ref_inner) break;
+      XCS_TRY(mr_gemm(h, FMT_BF16, true, D.rb, D.g, nullptr, nullptr, nullptr));
+      hipLaunchKernelGGL(mr_ref_dir_kernel, gs, bs, 0, h->stream, D, 0);
+    }
+    hipLaunchKernelGGL(mr_ref_update_kernel, gs, bs, 0, h->stream, D);
+  }
+  // final FP64 residual
+  XCS_TRY(mr_exact_resid(h, 0));
+  hipLaunchKernelGGL(mr_ref_resid_kernel, gs, bs, 0, h->stream, D, h->part, h->nblk);
+  HIP_TRY(hipGetLastError());
+  return XCS_OK;
+}
+
+static int mr_solve(xcs_mrhs *h, float *ms) {
+  MrhsDesc &D = h->D;
+  hipEvent_t ev[7];
+  for (int e = 0; e < 7; e++) HIP_TRY(hipEventCreate(&ev[e]));
+  HIP_TRY(hipEventRecord(ev[0], h->stream));
+  const dim3 gs(h->count), bs(MR_THREADS);
+  D.fmt = h->prec;
+  const float c_lp = h->prec == FMT_FP8 ? h->c8 : h->c16;
+  hipLaunchKernelGGL(mr_init0_kernel, gs, bs, 0, h->stream, D);
+  XCS_TRY(mr_gemm(h, D.fmt, true, D.rb, D.g, D.sr, nullptr, nullptr));
+  hipLaunchKernelGGL(mr_init1_kernel, gs, bs, 0, h->stream, D, c_lp);
+  XCS_TRY(mr_gemm(h, D.fmt, false, D.xb, D.ax, D.sx, nullptr, nullptr));
+  hipLaunchKernelGGL(mr_init2_kernel, gs, bs, 0, h->stream, D);
+  XCS_TRY(mr_gemm(h, D.fmt, true, D.rb, D.g, D.sr, nullptr, nullptr));
+  hipLaunchKernelGGL(mr_init3_kernel, gs, bs, 0, h->stream, D);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(ev[1], h->stream));
+  h->lp_iters[0] = h->lp_iters[1] = h->lp_iters[2] = 0;
+  XCS_TRY(mr_phase(h, false, &h->lp_iters[0]));
+  HIP_TRY(hipEventRecord(ev[2], h->stream));
+  if (h->prec == FMT_FP8) {
+    D.fmt = FMT_BF16;
+    hipLaunchKernelGGL(mr_switch_kernel, gs, bs, 0, h->stream, D, h->c16);
+    HIP_TRY(hipGetLastError());
+    XCS_TRY(mr_phase(h, false, &h->lp_iters[1]));
+  }
+  HIP_TRY(hipEventRecord(ev[3], h->stream));
+  XCS_TRY(mr_refine(h));
+  HIP_TRY(hipEventRecord(ev[4], h->stream));
+  // signals above tol after the refinement: CGIHT with the exact residual
+  hipLaunchKernelGGL(mr_mark_kernel, gs, dim3(64), 0, h->stream, D);
+  HIP_TRY(hipGetLastError());
+  D.fmt = FMT_BF16;
+  XCS_TRY(mr_phase(h, true, &h->lp_iters[2]));
+  HIP_TRY(hipEventRecord(ev[5], h->stream));
+  if (h->lp_iters[2] > 0) XCS_TRY(mr_refine(h));
+  HIP_TRY(hipEventRecord(ev[6], h->stream));
+  HIP_TRY(hipEventSynchronize(ev[6]));
+  for (int e = 0; e < 6; e++) HIP_TRY(hipEventElapsedTime(&h->phase_ms[e], ev[e], ev[e + 1]));
+  HIP_TRY(hipEventElapsedTime(&h->last_ms, ev[0], ev[6]));
+  for (int e = 0; e < 7; e++) (void)hipEventDestroy(ev[e]);
+  D.fmt = h->prec;
+  if (ms) *ms = h->last_ms;
+  return XCS_OK;
+}
+
+extern "C" {
+
+int xcs_mrhs_create(int m, int n, int k, int count, const char *precision,
+                    const xcs_options *opts, xcs_mrhs **out) {
+  if (!out || !opts) return set_err(XCS_EINVAL, "NULL argument");
+  *out = nullptr;
+  xcs_mrhs *h = new xcs_mrhs();
+  const int r = mr_create(h, m, n, k, count, precision, opts);
+  if (r != XCS_OK) {
+    delete h;
+    return r;
+  }
+  *out = h;
+  return XCS_OK;
+}
+
+int xcs_mrhs_solve(xcs_mrhs *h, float *ms, int *lp_iterations) {
+  if (!h) return set_err(XCS_EINVAL, "NULL handle");
+  HIP_TRY(hipSetDevice(h->dev));
+  XCS_TRY(mr_solve(h, ms));
+  if (lp_iterations)
+    for (int q = 0; q < 3; q++) lp_iterations[q] = h->lp_iters[q];
+  return XCS_OK;
+}
+
+int xcs_mrhs_results(xcs_mrhs *h, float *norms, int *iterations, double *resid_rel) {
+  if (!h) return set_err(XCS_EINVAL, "NULL handle");
+  HIP_TRY(hipSetDevice(h->dev));
+  const size_t S = (size_t)h->count, n = (size_t)h->n;
+  std::vector<XState> st(S);
+  std::vector<double> ref(4 * S);
+  HIP_TRY(hipMemcpy(st.data(), h->D.st, sizeof(XState) * S, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(ref.data(), h->D.ref, sizeof(double) * 4 * S, hipMemcpyDeviceToHost));
+  std::vector<float> x;
+  if (norms) {
+    x.resize(S * n);
+    HIP_TRY(hipMemcpy(x.data(), h->D.x, sizeof(float) * S * n, hipMemcpyDeviceToHost));
+  }
+  for (size_t s = 0; s < S; s++) {
+    if (norms) host_norms(h->x_true[s], x.data() + s * n, h->k, norms + 3 * s, nullptr);
+    if (iterations) iterations[s] = st[s].iter;
+    if (resid_rel) resid_rel[s] = ref[4 * s + 2] > 0 ? sqrt(ref[4 * s + 1] / ref[4 * s + 2]) : 0.0;
+  }
+  return XCS_OK;
+}
+
+int xcs_mrhs_x(xcs_mrhs *h, int s, float *x_out, float *x_true, float *y) {
+  if (!h || s < 0 || s >= h->count) return set_err(XCS_EINVAL, "bad signal index");
+  HIP_TRY(hipSetDevice(h->dev));
+  if (x_out)
+    HIP_TRY(hipMemcpy(x_out, h->D.x + (size_t)s * h->n, 4 * (size_t)h->n, hipMemcpyDeviceToHost));
+  if (x_true) memcpy(x_true, h->x_true[s].data(), 4 * (size_t)h->n);
+  if (y) HIP_TRY(hipMemcpy(y, h->D.y + (size_t)s * h->m, 4 * (size_t)h->m, hipMemcpyDeviceToHost));
+  return XCS_OK;
+}
+
+int xcs_mrhs_gemm_timed(xcs_mrhs *h, int transpose, const char *fmt, int reps, float *ms_per,
+                        double *flops) {
+  if (!h || reps <= 0 || !ms_per || !fmt) return set_err(XCS_EINVAL, "bad arguments");
+  const int f = strcmp(fmt, "fp8") ? FMT_BF16 : FMT_FP8;
+  if (f == FMT_FP8 && !h->a8cm) return set_err(XCS_EINVAL, "fp8 copies exist only for fp8 handles");
+  HIP_TRY(hipSetDevice(h->dev));
+  MrhsDesc &D = h->D;
+  const uint8_t *B = transpose ? D.rb : D.vb1;
+  float *C = transpose ? D.g : D.w1;
+  float *bs = transpose ? D.sr : D.sv1;
+  std::vector<float> ones((size_t)h->count, 1.0f);
+  HIP_TRY(hipMemcpy(bs, ones.data(), 4 * ones.size(), hipMemcpyHostToDevice));
+  XCS_TRY(mr_gemm(h, f, transpose != 0, B, C, bs, nullptr, nullptr));
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  HIP_TRY(hipEventRecord(e0, h->stream));
+  for (int r = 0; r < reps; r++) XCS_TRY(mr_gemm(h, f, transpose != 0, B, C, bs, nullptr, nullptr));
+  HIP_TRY(hipEventRecord(e1, h->stream));
+  HIP_TRY(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *ms_per = ms / reps;
+  if (flops) *flops = 2.0 * h->m * (double)h->n * h->count;
+  return XCS_OK;
+}
+
+int xcs_mrhs_fp32_solve(xcs_mrhs *h, int first, int cnt, int mode, float *ms_total, float *norms,
+                        int *iterations) {
+  if (!h || first < 0 || cnt <= 0 || first + cnt > h->count || !ms_total)
+    return set_err(XCS_EINVAL, "bad signal range");
+  if (mode != 0 && mode != 1) return set_err(XCS_EINVAL, "mode must be 0 (device) or 1 (plain)");
+  HIP_TRY(hipSetDevice(h->dev));
+  xcs_options o = h->opts;
+  o.mode = mode;
+  xcs_plan pl;
+  XCS_TRY(given_begin(&pl, o.gpu_number));
+  XCS_TRY(plan_alloc(&pl, ALG_CGIHT, ENS_GEN, h->k, h->m, h->n, 0, 0, &o));
+  if (pl.h.lda != h->m) return set_err(XCS_EINVAL, "internal: leading dimension");
+  xset(pl.h.A, h->A32);
+  std::vector<float> x((size_t)h->n);
+  float total = 0.f;
+  for (int q = 0; q < cnt; q++) {
+    const int s = first + q;
+    xset(pl.h.y, h->D.y + (size_t)s * h->m);
+    float ms = 0.f;
+    XCS_TRY(plan_run(&pl, &ms));
+    total += ms;
+    HIP_TRY(hipMemcpy(x.data(), pl.h.x, 4 * (size_t)h->n, hipMemcpyDeviceToHost));
+    if (norms) host_norms(h->x_true[s], x.data(), h->k, norms + 3 * (size_t)q, nullptr);
+    if (iterations) iterations[q] = pl.last.iter;
+  }
+  *ms_total = total;
+  return XCS_OK;
+}
+
+int xcs_mrhs_phase_ms(xcs_mrhs *h, float *ms6) {
+  if (!h || !ms6) return set_err(XCS_EINVAL, "NULL argument");
+  for (int e = 0; e < 6; e++) ms6[e] = h->phase_ms[e];
+  return XCS_OK;
+}
+
+void xcs_mrhs_destroy(xcs_mrhs *h) { delete h; }
+
+int xcs_gemm_tn(const char *fmt, int M, int N, int K, const float *A, const float *B, float *C,
+                float *A_q, float *B_q) {
+  if (!fmt || !A || !B || !C) return set_err(XCS_EINVAL, "NULL argument");
+  const int f = strcmp(fmt, "fp8") ? FMT_BF16 : FMT_FP8;
+  const int esz = f == FMT_BF16 ? 2 : 1;
+  if (M <= 0 || N <= 0 || K <= 0 || M % GT_M || ((long)K * esz) % GT_KB)
+    return set_err(XCS_EINVAL, "need M %% 128 == 0 and K * element bytes %% 128 == 0");
+  XCS_TRY(use_device(0));
+  xcs_mrhs h;  // scratch holder (allocations and stream)
+  HIP_TRY(hipStreamCreateWithFlags(&h.stream, hipStreamNonBlocking));
+  float *dA, *dB, *dC, *rs, *csb, *tmp;
+  uint8_t *qA, *qB;
+  XCS_TRY(h.alloc(&dA, (size_t)M * K));
+  XCS_TRY(h.alloc(&dB, (size_t)N * K));
+  XCS_TRY(h.alloc(&dC, (size_t)M * N));
+  XCS_TRY(h.alloc(&rs, (size_t)M));
+  XCS_TRY(h.alloc(&csb, (size_t)N));
+  XCS_TRY(h.alloc(&qA, (size_t)M * K * esz));
+  XCS_TRY(h.alloc(&qB, (size_t)N * K * esz));
+  XCS_TRY(h.alloc(&tmp, (size_t)std::max(M, N) * K));
+  HIP_TRY(hipMemcpyAsync(dA, A, 4 * (size_t)M * K, hipMemcpyHostToDevice, h.stream));
+  HIP_TRY(hipMemcpyAsync(dB, B, 4 * (size_t)N * K, hipMemcpyHostToDevice, h.stream));
+  if (f == FMT_FP8) {
+    hipLaunchKernelGGL(mr_colscale_kernel, dim3((M * 64 + 255) / 256), dim3(256), 0, h.stream, dA,
+                       K, M, rs);
+    hipLaunchKernelGGL(mr_colscale_kernel, dim3((N * 64 + 255) / 256), dim3(256), 0, h.stream, dB,
+                       K, N, csb);
+  }
+  hipLaunchKernelGGL(mr_quant_rows_kernel, dim3(1024), dim3(256), 0, h.stream, dA, M, K, f, rs, qA);
+  hipLaunchKernelGGL(mr_quant_rows_kernel, dim3(1024), dim3(256), 0, h.stream, dB, N, K, f, csb, qB);
+  HIP_TRY(hipGetLastError());
+  GemmArgs g{};
+  g.A = qA;
+  g.lda_bytes = (long)K * esz;
+  g.B = qB;
+  g.ldb_bytes = (long)K * esz;
+  g.C = dC;
+  g.ldc = M;
+  g.M = M;
+  g.K = K;
+  g.N = N;
+  g.ksplit = 1;
+  g.row_scale = f == FMT_FP8 ? rs : nullptr;
+  g.col_scale = f == FMT_FP8 ? csb : nullptr;
+  const dim3 grid(M / GT_M, (N + GT_N - 1) / GT_N, 1);
+  if (f == FMT_BF16)
+    hipLaunchKernelGGL(gemm_tn_kernel<FMT_BF16>, grid, dim3(GT_THREADS), 0, h.stream, g);
+  else
+    hipLaunchKernelGGL(gemm_tn_kernel<FMT_FP8>, grid, dim3(GT_THREADS), 0, h.stream, g);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(C, dC, 4 * (size_t)M * N, hipMemcpyDeviceToHost, h.stream));
+  if (A_q) {
+    hipLaunchKernelGGL(mr_decode_kernel, dim3(1024), dim3(256), 0, h.stream, qA, f, (long)M * K, K,
+                       rs, tmp);
+    HIP_TRY(hipMemcpyAsync(A_q, tmp, 4 * (size_t)M * K, hipMemcpyDeviceToHost, h.stream));
+    HIP_TRY(hipStreamSynchronize(h.stream));
+  }
+  if (B_q) {
+    hipLaunchKernelGGL(mr_decode_kernel, dim3(1024), dim3(256), 0, h.stream, qB, f, (long)N * K, K,
+                       csb, tmp);
+    HIP_TRY(hipMemcpyAsync(B_q, tmp, 4 * (size_t)N * K, hipMemcpyDeviceToHost, h.stream));
+  }
+  HIP_TRY(hipStreamSynchronize(h.stream));
+  return XCS_OK;
+}
+
+}  // extern "C"

@@ -143,6 +143,41 @@ void xcs_batch_destroy(xcs_batch *batch);
 /* Phase tick counters (10 ns; 32 entries) of builds made with -DXCS_PHASE_TIMING; zeros otherwise. */
 int xcs_batch_phase_ticks(xcs_batch *batch, unsigned long long *ticks32);
 
+/* ---- many right-hand sides: one gen A shared by many signals ---- */
+/* The reference's gaga_cs('CGIHT', 'gen', k, m, n, options) solves one signal per call
+ * (GAGA/gaga/gaga_gen.cu:41, CGIHT_S_gen at GAGA/gaga/algorithms.cu:3689-3942); this form
+ * solves `count` signals against one A in lock step, the products as MFMA GEMMs.
+ * A ~ N(0, 1/m) from key (opts->seed, opts->instance); signal s is k-sparse with key
+ * (opts->seed, opts->instance + 1 + s) and values by opts->vec_distribution; y = A x (FP32 A,
+ * FP64 sums).  precision: "bf16" (A in BF16) or "fp8" (A in OCP e4m3 with per-column scales,
+ * then BF16); both end with a support-restricted least-squares refinement (FP32 CG, BF16
+ * products, FP64 residual from the FP32 A).  Needs m % 128 == 0, n % 512 == 0. */
+typedef struct xcs_mrhs xcs_mrhs;
+int xcs_mrhs_create(int m, int n, int k, int count, const char *precision,
+                    const xcs_options *opts, xcs_mrhs **handle);
+/* *ms = device time of the whole solve; lp_iterations[3] = lock-step iterations issued in the
+ * FP8 (or BF16) phase, in the BF16 continuation (0 for "bf16"), and in the exact-residual
+ * phase that resumes the signals still above tol after the first refinement. */
+int xcs_mrhs_solve(xcs_mrhs *handle, float *ms, int *lp_iterations);
+/* norms[3*s..]: relative l1/l2/linf error against the planted signal; iterations[s]: CGIHT
+ * iterations; resid_rel[s] = ||y - A x|| / ||y|| in FP64 with the FP32 A. */
+int xcs_mrhs_results(xcs_mrhs *handle, float *norms, int *iterations, double *resid_rel);
+int xcs_mrhs_x(xcs_mrhs *handle, int signal, float *x_out, float *x_true, float *y);
+/* Time `reps` of one product over all signals: transpose = 1 for A^T R (M = n, K = m), 0 for
+ * A V (M = m, K = n, K split in 4); fmt "bf16" or "fp8"; *flops = 2 m n count. */
+int xcs_mrhs_gemm_timed(xcs_mrhs *handle, int transpose, const char *fmt, int reps,
+                        float *ms_per, double *flops);
+/* The same signals [first, first + cnt) solved one at a time by the FP32 single-instance
+ * engine on the resident A and y (mode 0: one cooperative launch per solve; 1: one outer
+ * iteration per launch, host-side stop test): *ms_total = summed device time,
+ * norms[3*q..] and iterations[q] per signal. */
+int xcs_mrhs_fp32_solve(xcs_mrhs *handle, int first, int cnt, int mode, float *ms_total,
+                        float *norms, int *iterations);
+/* Device ms of the last solve's phases: start, low-precision phase, BF16 continuation,
+ * refinement, exact-residual phase, second refinement. */
+int xcs_mrhs_phase_ms(xcs_mrhs *handle, float *ms6);
+void xcs_mrhs_destroy(xcs_mrhs *handle);
+
 /* ---- kernel-level entry points (host buffers; used by parity tests) ---- */
 /* One FindSupportSet call (functions.cu:397-436) on the device:
  * in/out state: *min_val, *alpha, *max_bin, *k_bin; out: bins[n], *sum
@@ -157,6 +192,12 @@ int xcs_apply(const char *ens, int transpose, int m, int n, const float *A, cons
               const int *rows, const float *vals, const int *dct_rows, const float *in,
               float *out);
 /* Device generator (Philox4x32-10): gen A column-major m x n with N(0, 1/m) entries. */
+/* C[c*M + r] = sum_k q(A)[r, k] q(B)[c, k] on the MFMA GEMM (A: M x K row-major, B: N x K
+ * row-major, host FP32).  fmt "bf16": q = RNE to bf16; "fp8": q = e4m3(v / s) * s with s the
+ * row's max |v| / 448.  A_q, B_q (nullable) receive the dequantized operands.  Needs
+ * M % 128 == 0 and K * element bytes % 128 == 0. */
+int xcs_gemm_tn(const char *fmt, int M, int N, int K, const float *A, const float *B, float *C,
+                float *A_q, float *B_q);
 int xcs_generate_gen(int m, int n, unsigned int seed, unsigned int instance, float *A);
 
 const char *xcs_last_error(void);

@@ -1,0 +1,122 @@
+"""Many right-hand sides on the GPU: the MFMA GEMMs against float64 products
+of the quantized operands, the quantizers against numpy roundings, and the
+lock-step mixed-precision CGIHT against the planted signals and the FP64
+oracle."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SEED = 20260101
+
+
+from numerics import bf16_rne, e4m3_rne  # noqa: E402
+
+
+def _bound(Aq, Bq, K):
+    # FP32 accumulation: |error| <= K * 2^-23 * sum |a||b| (generous constant)
+    return K * 2.0 ** -23 * (np.abs(Aq).astype(np.float64) @ np.abs(Bq).astype(np.float64).T) + 1e-30
+
+
+@pytest.mark.parametrize("M,N,K", [(128, 128, 64), (256, 72, 512), (384, 200, 1024)])
+def test_gemm_bf16(M, N, K):
+    import gaga_amd as ga
+    rng = np.random.default_rng(M + N + K)
+    A = rng.standard_normal((M, K)).astype(np.float32)
+    B = rng.standard_normal((N, K)).astype(np.float32)
+    C, Aq, Bq = ga.gemm_tn("bf16", A, B)
+    np.testing.assert_array_equal(Aq, bf16_rne(A))
+    np.testing.assert_array_equal(Bq, bf16_rne(B))
+    ref = Aq.astype(np.float64) @ Bq.astype(np.float64).T
+    assert (np.abs(C - ref) <= _bound(Aq, Bq, K)).all()
+    # a transposed operand or a wrong K pairing would be far off
+    assert np.abs(C - ref).max() < 1e-3 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("M,N,K", [(128, 128, 128), (256, 72, 512), (384, 200, 1024)])
+def test_gemm_fp8(M, N, K):
+    import gaga_amd as ga
+    rng = np.random.default_rng(7 * M + N + K)
+    A = rng.standard_normal((M, K)).astype(np.float32)
+    B = (rng.standard_normal((N, K)) * np.exp(rng.standard_normal((N, 1)))).astype(np.float32)
+    C, Aq, Bq = ga.gemm_tn("fp8", A, B)
+    for X, Xq in ((A, Aq), (B, Bq)):
+        s = (np.abs(X).max(axis=1) / np.float32(448.0)).astype(np.float32)
+        v = (X / s[:, None]).astype(np.float32)
+        want = (e4m3_rne(v).astype(np.float32) * s[:, None]).astype(np.float32)
+        np.testing.assert_array_equal(Xq, want)
+    ref = Aq.astype(np.float64) @ Bq.astype(np.float64).T
+    assert (np.abs(C - ref) <= _bound(Aq, Bq, K)).all()
+
+
+def test_gemm_special_values():
+    """Zero rows, exact small integers, a single nonzero: exact results."""
+    import gaga_amd as ga
+    M, N, K = 128, 40, 128
+    A = np.zeros((M, K), np.float32)
+    B = np.zeros((N, K), np.float32)
+    A[3, 5] = 2.0
+    B[7, 5] = -3.0
+    A[100, :] = 1.0
+    B[0, :] = 1.0
+    want = A.astype(np.float64) @ B.astype(np.float64).T
+    C, _, _ = ga.gemm_tn("bf16", A, B)
+    np.testing.assert_array_equal(C, want)
+    # fp8: the row and column scales are inexact in FP32, the rest is exact
+    C, _, _ = ga.gemm_tn("fp8", A, B)
+    np.testing.assert_allclose(C, want, rtol=1e-6, atol=0)
+
+
+@pytest.mark.parametrize("precision", ["bf16", "fp8"])
+def test_many_rhs_recovers(precision, oracle):
+    import gaga_amd as ga
+    m, n, k, count = 512, 2048, 40, 96
+    opts = ga.options(tol=1e-5, seed=SEED, vecDistribution="gaussian", instance=5)
+    R = ga.ManyRHS(m, n, k, count, precision, opts)
+    ms, lp = R.solve()
+    r = R.results()
+    ok = r["norms"][:, 2] < 1e-3
+    print("%s: %.1f ms, lp iterations %s, success %d/%d, resid_rel median %.2e"
+          % (precision, ms, lp, ok.sum(), count, np.median(r["resid_rel"])))
+    assert ok.mean() >= 0.95
+    # refined solutions sit at the FP32 floor of the residual
+    assert np.median(r["resid_rel"][ok]) < 1e-5
+    # against the FP64 oracle on the same problems (same generator, key (SEED, 5))
+    P = oracle.make_problem("gen", m, n, SEED, 5)
+    for s in range(count):
+        # the refinement solves on the k largest entries: at most k nonzeros remain
+        assert (R.x(s)[0] != 0).sum() <= k
+    for s in (0, 1, count - 1):
+        x, xt, y = R.x(s)
+        xo, res, _, _ = oracle.solve("CGIHT", P, y, k, tol=1e-5, precision=64)
+        if np.max(np.abs(xo - xt)) / np.max(np.abs(xt)) < 1e-3:
+            assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-4
+    R.close()
+
+
+def test_many_rhs_gemm_timing_entry():
+    import gaga_amd as ga
+    R = ga.ManyRHS(512, 2048, 40, 64, "fp8", ga.options(seed=SEED))
+    for t in (0, 1):
+        for fmt in ("bf16", "fp8"):
+            ms, fl = R.gemm_timed(t, fmt, reps=3)
+            assert ms > 0 and fl == 2.0 * 512 * 2048 * 64
+    R.close()
+
+
+def test_many_rhs_fp32_reference_form():
+    """The FP32 one-instance-per-launch engine on the same resident A and y
+    recovers the same signals (device and plain forms agree bit for bit)."""
+    import gaga_amd as ga
+    m, n, k, count = 512, 2048, 40, 16
+    R = ga.ManyRHS(m, n, k, count, "bf16", ga.options(tol=1e-5, seed=SEED,
+                                                      vecDistribution="gaussian", instance=6))
+    R.solve()
+    mixed = R.results()["norms"][:4, 2]
+    ms_d, nd, itd = R.fp32_solve(0, 4, "device")
+    ms_p, npl, itp = R.fp32_solve(0, 4, "plain")
+    np.testing.assert_array_equal(nd, npl)
+    np.testing.assert_array_equal(itd, itp)
+    assert ms_d > 0 and ms_p > 0
+    assert ((nd[:, 2] < 1e-3) == (mixed < 1e-3)).all()
+    R.close()
