@@ -11,9 +11,15 @@ per second of timed wall clock, inputs already resident in HBM.
 Also reported (same JSON line):
   roofline      the solve kernel's algorithmic operator bytes per launch /
                 its average launch duration (HIP events), against 8 TB/s
-  matvec        the dense A^T r product alone on one instance (whole grid)
+  matvec        the dense A^T r product of the batched engine alone (one
+                workgroup per instance, all instances of the batch)
   forms         plain (one instance per launch, host-stepped iterations) and
                 single (one instance per cooperative launch) problems/s
+  sweep         the 4096-instance phase-transition sweep (gen + smv, n up to
+                8192, uneven sizes) in one launch, autotuned geometry
+  many_rhs      512 signals sharing one 4096 x 16384 A: BF16 and FP8 MFMA
+                lock-step CGIHT with refinement, against the FP32
+                one-instance-per-launch form on the same signals
   cpu_baseline  the FP64 CPU oracle (a C restatement of the reference's
                 algorithm) on a bounded sample of the same instances
 """
@@ -67,6 +73,86 @@ def cpu_baseline(sample):
                                                                                  t_total))
 
 
+def run_sweep():
+    """4096 uneven CGIHT instances (gen and smv, n 1024..8192, delta x rho grid)
+    in one persistent launch; tol 1e-4 and maxiter 500 as the reference's
+    transition driver (GAGA/CGIHTcode/code/generate_data_transition.m:60-72)."""
+    import gaga_amd as ga
+    from gaga_amd import sweep as sw
+    specs = sw.grid_specs()
+    t0 = time.perf_counter()
+    B = ga.Batch("CGIHT", specs, ga.options(tol=1e-4, maxiter=500, seed=SEED,
+                                            vecDistribution="binary"))
+    create_s = time.perf_counter() - t0
+    team, block, probe_ms = B.tune(probe_iters=32, cache=os.path.join(ROOT, "gaga_amd",
+                                                                     "tune_cache.txt"))
+    B.solve()  # warm
+    sync()
+    t0 = time.perf_counter()
+    ms = B.solve()
+    wall = time.perf_counter() - t0
+    r = B.results()
+    ok = r["norms"][:, 2] < SUCCESS
+    byt = B.bytes()
+    B.close()
+    log("sweep: %.1f ms, %d / %d recovered" % (ms, ok.sum(), len(specs)))
+    return dict(workload="CGIHT sweep: gen+smv(p=7), n in {1024..8192}, 16 delta x 16 rho x 2 seeds",
+                instances=len(specs), solve_ms=ms, wall_s=wall, recovered=int(ok.sum()),
+                problems_per_s=len(specs) / wall, recovered_per_s=int(ok.sum()) / wall,
+                algorithmic_GBs=byt / (ms * 1e-3) / 1e9, hbm_frac=byt / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                tuned={"team_size": team, "block": block, "probe_ms": probe_ms},
+                generation_s=create_s)
+
+
+def run_many_rhs():
+    """512 signals (k = 400, Gaussian values) sharing one gen A of 4096 x 16384."""
+    import gaga_amd as ga
+    m, n, k, S = 4096, 16384, 400, 512
+    out = {"workload": "CGIHT many-rhs: gen %dx%d, %d signals, k=%d gaussian, tol 1e-5" % (m, n, S, k)}
+    base = None
+    for prec in ("bf16", "fp8"):
+        R = ga.ManyRHS(m, n, k, S, prec, ga.options(tol=TOL, seed=SEED, vecDistribution="gaussian",
+                                                    instance=9))
+        R.solve()
+        sync()
+        t0 = time.perf_counter()
+        ms, lp = R.solve()
+        wall = time.perf_counter() - t0
+        r = R.results()
+        ok = r["norms"][:, 2] < SUCCESS
+        gemm = {}
+        for t, name in ((1, "AT_R"), (0, "A_V")):
+            g_ms, fl = R.gemm_timed(t, prec, reps=20)
+            peak = 2500.0 if prec == "bf16" else 5000.0
+            gemm[name] = dict(ms=g_ms, tflops=fl / g_ms / 1e9, peak_tflops=peak,
+                              frac=fl / g_ms / 1e9 / peak)
+        out[prec] = dict(solve_ms=ms, wall_s=wall, recovered=int(ok.sum()),
+                         problems_per_s=S / wall, recovered_per_s=int(ok.sum()) / wall,
+                         lockstep_iterations=[int(v) for v in lp],
+                         phase_ms=[float(v) for v in R.phase_ms()],
+                         rel_linf_median=float(np.median(r["norms"][:, 2])),
+                         resid_rel_median=float(np.median(r["resid_rel"])), gemm=gemm)
+        log("many-rhs %s: %.1f ms, %d / %d recovered" % (prec, ms, ok.sum(), S))
+        if prec == "bf16":
+            sample = 16
+            res = {}
+            for mode in ("device", "plain"):
+                t0 = time.perf_counter()
+                ms32, nr, it = R.fp32_solve(0, sample, mode)
+                w32 = time.perf_counter() - t0
+                ok32 = nr[:, 2] < SUCCESS
+                res[mode] = dict(signals=sample, wall_s=w32, recovered=int(ok32.sum()),
+                                 problems_per_s=sample / w32,
+                                 recovered_per_s=int(ok32.sum()) / w32,
+                                 iterations_mean=float(it.mean()))
+            base = res
+            out["fp32_one_instance_per_launch"] = res
+        R.close()
+    best = max(out["bf16"]["recovered_per_s"], out["fp8"]["recovered_per_s"])
+    out["speedup_vs_fp32_plain"] = best / max(base["plain"]["recovered_per_s"], 1e-9)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,6 +164,8 @@ def main():
     ap.add_argument("--plain-instances", type=int, default=8)
     ap.add_argument("--cpu-sample", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--no-mrhs", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -112,12 +200,13 @@ def main():
     achieved = bytes_per_launch / (mean_ms * 1e-3) / 1e9
     iters = r["iters"]
     linf = r["norms"][:, 2]
+    # dense A^T r alone: every instance of the batch, one workgroup each
+    mv_ms, mv_bytes = B.apply_timed(10)
     B.close()
 
-    # dense A^T r alone (whole grid on one resident instance)
     pl = ga.Plan("CGIHT", "gen", K, M, N, 0, ga.options(tol=TOL, seed=SEED, instance=0,
                                                        vecDistribution="gaussian"))
-    mv_ms, mv_bytes = pl.apply_timed(1, 50)
+    mv1_ms, mv1_bytes = pl.apply_timed(1, 50)
     # single-instance forms: one instance per launch (device loop) and plain
     # (one outer iteration per launch, host-side stop test)
     forms = {}
@@ -137,6 +226,8 @@ def main():
         forms["single_" + mode] = dict(problems_per_s=ok / tsum, instances=args.plain_instances)
     pl.close()
 
+    sweep = None if args.no_sweep else run_sweep()
+    mrhs = None if args.no_mrhs else run_many_rhs()
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_sample)
 
     out = {
@@ -176,14 +267,17 @@ def main():
             "avg_launch_ms": mean_ms,
         },
         "matvec": {
-            "kernel": "apply_coop_kernel (gen A^T r, one instance, whole grid)",
+            "kernel": "apply_batch_kernel (gen A^T r, %d instances, one workgroup each)" % args.batch,
             "achieved": mv_bytes / (mv_ms * 1e-3) / 1e9,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": mv_bytes / (mv_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "bytes": mv_bytes,
             "avg_ms": mv_ms,
+            "single_instance_whole_grid_GBs": mv1_bytes / (mv1_ms * 1e-3) / 1e9,
         },
+        "sweep": sweep,
+        "many_rhs": mrhs,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

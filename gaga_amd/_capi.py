@@ -17,7 +17,7 @@ EXPORTS = (
     "xcs_batch_x", "xcs_batch_info", "xcs_batch_destroy", "xcs_batch_phase_ticks",
     "xcs_find_support_set", "xcs_apply", "xcs_generate_gen", "xcs_gemm_tn",
     "xcs_mrhs_create", "xcs_mrhs_solve", "xcs_mrhs_results", "xcs_mrhs_x",
-    "xcs_mrhs_gemm_timed", "xcs_mrhs_phase_ms", "xcs_mrhs_fp32_solve", "xcs_mrhs_destroy",
+    "xcs_mrhs_gemm_timed", "xcs_mrhs_phase_ms", "xcs_mrhs_fp32_solve", "xcs_mrhs_configure", "xcs_mrhs_destroy",
     "xcs_last_error", "xcs_version",
 )
 
@@ -80,6 +80,7 @@ def load():
         L.xcs_mrhs_x.argtypes = [_P, _I, _P, _P, _P]
         L.xcs_mrhs_gemm_timed.argtypes = [_P, _I, _C, _I, _P, _P]
         L.xcs_mrhs_phase_ms.argtypes = [_P, _P]
+        L.xcs_mrhs_configure.argtypes = [_P, ctypes.c_float, ctypes.c_float, _I, _I, _I]
         L.xcs_mrhs_fp32_solve.argtypes = [_P, _I, _I, _I, _P, _P, _P]
         L.xcs_mrhs_destroy.argtypes = [_P]
         L.xcs_mrhs_destroy.restype = None
@@ -408,6 +409,11 @@ class ManyRHS:
         _ck(load().xcs_mrhs_fp32_solve(self._h, int(first), int(cnt), 1 if mode == "plain" else 0,
                                        ctypes.byref(ms), _p(norms), _p(it)))
         return ms.value, norms, it
+
+    def configure(self, c_bf16=2.0 ** -7, c_fp8=2.0 ** -3, corrections=6, ref_outer=2,
+                  ref_inner=4):
+        _ck(load().xcs_mrhs_configure(self._h, float(c_bf16), float(c_fp8), int(corrections),
+                                      int(ref_outer), int(ref_inner)))
 
     def phase_ms(self):
         out = np.zeros(6, np.float32)

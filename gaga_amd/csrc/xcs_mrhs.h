@@ -14,16 +14,24 @@
 // Precision: the products read A in BF16 (RNE) or OCP FP8 e4m3 with one scale
 // per column (s_j = max_i |A_ij| / 448) and the signal operand in the same
 // format (FP8: one scale per signal), accumulating in FP32; the vectors are
-// FP32 and the norms FP64.  The low-precision phase stops at its own floor
-// (tolerance c * ||y|| with c = 2^-7 for BF16, 2^-3 for FP8; FP8 continues in
-// BF16 from there).  Then, on the support found, a least-squares refinement:
-// the residual r = y - A_S x_S is formed from the FP32 A with FP64 sums, the
-// correction solves min ||r - A_S d|| by FP32 CG whose products are BF16
-// GEMMs, and x_S += d (mixed-precision iterative refinement).
+// FP32 and the norms FP64.  The low-precision CGIHT stops near its floor
+// (tolerance c * ||y||, c = 2^-7 for BF16 and 2^-3 for FP8; FP8 continues in
+// BF16 from there).  Then:
+//   refinement  on the k largest entries of x, least squares by mixed-
+//               precision iterative refinement: r = y - A_S x_S from the FP32
+//               A with FP64 sums, the correction min ||r - A_S d|| by FP32 CG
+//               whose products are BF16 GEMMs, x_S += d;
+//   corrections for signals still above tol: g = A^T r (exact r), the NIHT
+//               step on the current support, x += mu g, and the refinement
+//               again, which keeps the k largest entries (hard thresholding
+//               pursuit with an exact k-term threshold).
+// The FP64 residual of the FP32 A decides success against tol, as the
+// reference's stopping rule does.
 
 namespace xcs {
 
-constexpr int MR_THREADS = 256;
+constexpr int MR_THREADS = 256;  // row-parallel kernels (one row or 4 rows per thread)
+constexpr int MR_VT = 1024;      // per-signal vector kernels (one workgroup per signal)
 constexpr int MR_KSPLIT = 4;  // K splits of the A * V products (K = n)
 
 struct MrhsDesc {
@@ -122,14 +130,14 @@ __device__ __forceinline__ const float *mr_pre(const MrhsDesc &D) {
   (void)t;
 
 // y as the first right-hand side operand (A^T y).
-__global__ void __launch_bounds__(MR_THREADS) mr_init0_kernel(MrhsDesc D) {
+__global__ void __launch_bounds__(MR_VT) mr_init0_kernel(MrhsDesc D) {
   const int s = blockIdx.x;
   Team t{0, 1, nullptr, 0, nullptr};
   mr_quant(t, D, D.y + (long)s * D.m, nullptr, D.m, D.rb, D.sr, s);
 }
 
 // x = A^T y, first support (algorithms.cu:3752-3790 with solve_instance's init).
-__global__ void __launch_bounds__(MR_THREADS) mr_init1_kernel(MrhsDesc D, float c_lp) {
+__global__ void __launch_bounds__(MR_VT) mr_init1_kernel(MrhsDesc D, float c_lp) {
   MR_PROLOGUE(D.tol);
   XState S;
   __builtin_memset(&S, 0, sizeof(S));
@@ -186,7 +194,7 @@ __global__ void __launch_bounds__(MR_THREADS) mr_init1_kernel(MrhsDesc D, float 
 }
 
 // r^0 = y - A x (algorithms.cu:3791-3800).
-__global__ void __launch_bounds__(MR_THREADS) mr_init2_kernel(MrhsDesc D) {
+__global__ void __launch_bounds__(MR_VT) mr_init2_kernel(MrhsDesc D) {
   MR_PROLOGUE(D.tol);
   if (D.stage[s] != 0) return;
   XState S = *P.st;
@@ -207,7 +215,7 @@ __global__ void __launch_bounds__(MR_THREADS) mr_init2_kernel(MrhsDesc D) {
 }
 
 // first gradient and its restriction (algorithms.cu:3801-3808); loop entry test.
-__global__ void __launch_bounds__(MR_THREADS) mr_init3_kernel(MrhsDesc D) {
+__global__ void __launch_bounds__(MR_VT) mr_init3_kernel(MrhsDesc D) {
   MR_PROLOGUE(D.tol_lp[blockIdx.x]);
   if (D.stage[s] != 0) return;
   XState S = *P.st;
@@ -267,7 +275,7 @@ __global__ void __launch_bounds__(MR_THREADS) mr_list_kernel(MrhsDesc D) {
 
 // Step part 1: r = y - A x (into resid for the restricted step, resid_update
 // for the unrestricted one), ||r|| into the history.
-__global__ void __launch_bounds__(MR_THREADS) mr_k1_kernel(MrhsDesc D) {
+__global__ void __launch_bounds__(MR_VT) mr_k1_kernel(MrhsDesc D) {
   MR_PROLOGUE(D.tol_lp[blockIdx.x]);
   if (D.stage[s] != 0) return;
   XState S = *P.st;
@@ -292,7 +300,7 @@ __global__ void __launch_bounds__(MR_THREADS) mr_k1_kernel(MrhsDesc D) {
 // whose restriction is the next product's operand (functions.cu:1004-1060);
 // unrestricted: the old restricted direction and the restricted gradient
 // are the operands (functions.cu:1107-1160).
-__global__ void __launch_bounds__(MR_THREADS) mr_k2_kernel(MrhsDesc D) {
+__global__ void __launch_bounds__(MR_VT) mr_k2_kernel(MrhsDesc D) {
   MR_PROLOGUE(D.tol_lp[blockIdx.x]);
   if (D.stage[s] != 0) return;
   XState S = *P.st;
@@ -349,7 +357,7 @@ __global__ void __launch_bounds__(MR_THREADS) mr_k2_kernel(MrhsDesc D) {
 
 // Step part 3 (after the A p products): step length, x update, support
 // update and stopping rules (algorithms.cu:3835-3942).
-__global__ void __launch_bounds__(MR_THREADS) mr_k3_kernel(MrhsDesc D) {
+__global__ void __launch_bounds__(MR_VT) mr_k3_kernel(MrhsDesc D) {
   MR_PROLOGUE(D.tol_lp[blockIdx.x]);
   if (D.stage[s] != 0) return;
   XState S = *P.st;
@@ -460,7 +468,7 @@ __global__ void __launch_bounds__(MR_THREADS) mr_k3_kernel(MrhsDesc D) {
 }
 
 // FP8 phase finished: continue every signal in BF16 (D.fmt is BF16 here).
-__global__ void __launch_bounds__(MR_THREADS) mr_switch_kernel(MrhsDesc D, float c_lp) {
+__global__ void __launch_bounds__(MR_VT) mr_switch_kernel(MrhsDesc D, float c_lp) {
   MR_PROLOGUE(D.tol);
   XState S = *P.st;
   if (S.max_value == 0.f) return;  // A^T y = 0: nothing to continue
@@ -478,20 +486,20 @@ __global__ void __launch_bounds__(MR_THREADS) mr_switch_kernel(MrhsDesc D, float
 }
 
 // ------------------------------------------------------------ refinement ---
-// Support lists.  topk = 0: the nonzeros of x, for the active signals (the
-// exact-residual phase).  topk = 1: the refinement's support, the k largest
+// Support lists.  topk = 0: the nonzeros of x (active signals only).
+// topk = 1: the refinement's support, the k largest
 // |x_j| (all nonzeros when there are at most k; ties at the k-th magnitude
 // taken in index order); x is cut to it and bins2 = 1 marks it.  The k-th
 // magnitude is found by a bitwise search on the IEEE bits of |x| (monotone
 // for non-negative floats).
-__global__ void __launch_bounds__(MR_THREADS) mr_ref_supp_kernel(MrhsDesc D, int topk) {
+__global__ void __launch_bounds__(MR_VT) mr_ref_supp_kernel(MrhsDesc D, int topk) {
   const int s = blockIdx.x;
   if (!topk && D.stage[s] != 0) return;
   const long n = D.n;
   float *x = D.x + s * n;
   int *mask = D.bins2 + s * n, *sup = D.supp + s * n;
   __shared__ int base;
-  __shared__ int cnt_s[MR_THREADS / 64];
+  __shared__ int cnt_s[MAX_WAVES];
   auto count_ge = [&](uint32_t thr) {
     int c = 0;
     for (int j = threadIdx.x; j < D.n; j += blockDim.x) {
@@ -544,66 +552,90 @@ __global__ void __launch_bounds__(MR_THREADS) mr_ref_supp_kernel(MrhsDesc D, int
 }
 
 // r = y - A_S x_S from the FP32 A with FP64 sums over the support list of
-// each selected signal.  Pass 1: one workgroup per (256-row block, signal,
-// slice z of MR_SLICES), one row per thread, over the support entries
-// c = z*256 .. (mod MR_SLICES*256) staged through LDS; FP64 row sums into
-// rowpart.  Pass 2 (mr_resid_sum_kernel): slices summed in order, r and the
-// block's FP64 partial of ||r||^2 (part[s * nblk + block]).
-// which: 0 every signal, 1 the active ones (stage 0; r into resid or, for an
-// unrestricted step, resid_update).
-constexpr int MR_SLICES = 8;
-__global__ void __launch_bounds__(MR_THREADS) mr_exact_resid_kernel(MrhsDesc D, int which,
-                                                                    double *rowpart) {
-  __shared__ int sj[MR_THREADS];
-  __shared__ float sv[MR_THREADS];
-  const int s = blockIdx.y, z = blockIdx.z;
-  if (which == 1 && D.stage[s] != 0) return;
+// each selected signal.  Pass 1: one workgroup per (slice z of MR_SLICES,
+// signal) streams whole columns (contiguous m floats): support chunks z,
+// z + MR_SLICES, ... of MR_CHUNK entries are staged through LDS, and each
+// thread owns rows 4t + 1024q (float4 loads), keeping FP64 row sums in
+// registers; the slice's sums go to rowpart.  Pass 2 (mr_resid_sum_kernel):
+// slices summed in order, r and the FP64 partial of ||r||^2 of each 256-row
+// block (part[s * nblk + block]).  Needs m <= 1024 * MR_MAXQ.
+constexpr int MR_SLICES = 64;
+constexpr int MR_CHUNK = 16;
+constexpr int MR_MAXQ = 8;
+__global__ void __launch_bounds__(MR_THREADS) mr_exact_resid_kernel(MrhsDesc D, double *rowpart) {
+  __shared__ int sj[MR_CHUNK];
+  __shared__ float sv[MR_CHUNK];
+  const int z = blockIdx.x, s = blockIdx.y;
+  const int slices = MR_SLICES;
   const long n = D.n, m = D.m;
+  const int cnt = D.supp_cnt[s];
+  if (z * MR_CHUNK >= cnt) return;
   const float *x = D.x + s * n;
   const int *sup = D.supp + s * n;
-  const int cnt = D.supp_cnt[s];
-  const int i = blockIdx.x * MR_THREADS + threadIdx.x;
-  double acc = 0.0;
-  for (int c0 = z * MR_THREADS; c0 < cnt; c0 += MR_SLICES * MR_THREADS) {
+  const int nq = (D.m + 4 * MR_THREADS - 1) / (4 * MR_THREADS);
+  double acc[MR_MAXQ][4];
+#pragma unroll
+  for (int q = 0; q < MR_MAXQ; q++)
+#pragma unroll
+    for (int e = 0; e < 4; e++) acc[q][e] = 0.0;
+  for (int c0 = z * MR_CHUNK; c0 < cnt; c0 += slices * MR_CHUNK) {
     __syncthreads();
-    if (c0 + (int)threadIdx.x < cnt) {
+    if (threadIdx.x < MR_CHUNK && c0 + (int)threadIdx.x < cnt) {
       const int j = sup[c0 + threadIdx.x];
       sj[threadIdx.x] = j;
       sv[threadIdx.x] = x[j];
     }
     __syncthreads();
-    const int c1 = min(MR_THREADS, cnt - c0);
-    if (i < D.m) {
-      int c = 0;
-      for (; c + 4 <= c1; c += 4) {
-        const float a0 = D.A32[(long)sj[c] * m + i], a1 = D.A32[(long)sj[c + 1] * m + i];
-        const float a2 = D.A32[(long)sj[c + 2] * m + i], a3 = D.A32[(long)sj[c + 3] * m + i];
-        acc = fma((double)a0, (double)sv[c], acc);
-        acc = fma((double)a1, (double)sv[c + 1], acc);
-        acc = fma((double)a2, (double)sv[c + 2], acc);
-        acc = fma((double)a3, (double)sv[c + 3], acc);
+    const int c1 = min(MR_CHUNK, cnt - c0);
+    for (int c = 0; c < c1; c++) {
+      const float *col = D.A32 + (long)sj[c] * m;
+      const double xv = (double)sv[c];
+      float4 v[MR_MAXQ];
+#pragma unroll
+      for (int q = 0; q < MR_MAXQ; q++) {
+        const int i = 4 * (int)threadIdx.x + 4 * MR_THREADS * q;
+        if (q < nq && i < D.m) v[q] = *(const float4 *)(col + i);
       }
-      for (; c < c1; c++) acc = fma((double)D.A32[(long)sj[c] * m + i], (double)sv[c], acc);
+#pragma unroll
+      for (int q = 0; q < MR_MAXQ; q++) {
+        const int i = 4 * (int)threadIdx.x + 4 * MR_THREADS * q;
+        if (q < nq && i < D.m) {
+          acc[q][0] = fma((double)v[q].x, xv, acc[q][0]);
+          acc[q][1] = fma((double)v[q].y, xv, acc[q][1]);
+          acc[q][2] = fma((double)v[q].z, xv, acc[q][2]);
+          acc[q][3] = fma((double)v[q].w, xv, acc[q][3]);
+        }
+      }
     }
   }
-  if (i < D.m) rowpart[((long)s * MR_SLICES + z) * m + i] = acc;
+  double *o = rowpart + ((long)s * slices + z) * m;
+#pragma unroll
+  for (int q = 0; q < MR_MAXQ; q++) {
+    const int i = 4 * (int)threadIdx.x + 4 * MR_THREADS * q;
+    if (q < nq && i < D.m) {
+      o[i] = acc[q][0];
+      o[i + 1] = acc[q][1];
+      o[i + 2] = acc[q][2];
+      o[i + 3] = acc[q][3];
+    }
+  }
 }
 
-__global__ void __launch_bounds__(MR_THREADS) mr_resid_sum_kernel(MrhsDesc D, int which,
+__global__ void __launch_bounds__(MR_THREADS) mr_resid_sum_kernel(MrhsDesc D,
                                                                   const double *rowpart,
                                                                   double *part) {
   __shared__ double red[MR_THREADS / 64];
-  const int s = blockIdx.y;
-  if (which == 1 && D.stage[s] != 0) return;
+  const int s = blockIdx.y, slices = MR_SLICES;
   const long m = D.m;
+  const int cnt = D.supp_cnt[s];
+  const int used = min(slices, (cnt + MR_CHUNK - 1) / MR_CHUNK);  // slices that wrote
   const int i = blockIdx.x * MR_THREADS + threadIdx.x;
   double e2 = 0.0;
   if (i < D.m) {
     double a = 0.0;
-    for (int z = 0; z < MR_SLICES; z++) a += rowpart[((long)s * MR_SLICES + z) * m + i];
+    for (int z = 0; z < used; z++) a += rowpart[((long)s * slices + z) * m + i];
     const double ri = (double)D.y[s * m + i] - a;
-    float *r = (which == 1 && D.mode[s]) ? D.resid_update : D.resid;
-    r[s * m + i] = (float)ri;
+    D.resid[s * m + i] = (float)ri;
     e2 = ri * ri;
   }
   e2 = wave_sum(e2);
@@ -622,9 +654,9 @@ __device__ __forceinline__ double mr_part_sum(const double *part, int s, int nbl
   return t;
 }
 
-// Refinement residual (after mr_exact_resid_kernel, which 0): ||r||^2 into
+// Refinement residual (after mr_exact_resid_kernel): ||r||^2 into
 // ref; d = 0; r as the next A^T operand (BF16).
-__global__ void __launch_bounds__(MR_THREADS) mr_ref_resid_kernel(MrhsDesc D, const double *part,
+__global__ void __launch_bounds__(MR_VT) mr_ref_resid_kernel(MrhsDesc D, const double *part,
                                                                   int nblk) {
   const int s = blockIdx.x;
   const long n = D.n, m = D.m;
@@ -636,48 +668,57 @@ __global__ void __launch_bounds__(MR_THREADS) mr_ref_resid_kernel(MrhsDesc D, co
   mr_quant(t, D, D.resid + s * m, nullptr, D.m, D.rb, D.sr, s);
 }
 
-// Step part 1 with the exact residual (after mr_exact_resid_kernel, which 1):
-// ||r|| into the history, r as the next A^T operand.
-__global__ void __launch_bounds__(MR_THREADS) mr_k1x_kernel(MrhsDesc D, const double *part,
-                                                            int nblk) {
-  MR_PROLOGUE(D.tol_lp[blockIdx.x]);
-  if (D.stage[s] != 0) return;
-  XState S = *P.st;
-  const float *r = D.mode[s] ? (const float *)P.resid_update : (const float *)P.resid;
-  push16(S.prev, (float)sqrt(mr_part_sum(part, s, nblk)));
-  mr_quant(t, D, r, nullptr, P.m, D.rb, D.sr, s);
-  if (threadIdx.x == 0) *P.st = S;
+// Support correction on the exact residual (after a refinement, whose final
+// pass left r in resid and its BF16 copy in rb, and g = A^T r in D.g): the
+// signals with ||r|| > tol are active (stage 0); g restricted to the current
+// support (bins2) is the operand of the step-size product.
+__global__ void __launch_bounds__(MR_VT) mr_htp_grad_kernel(MrhsDesc D) {
+  const int s = blockIdx.x;
+  const long n = D.n;
+  const double rn = sqrt(D.ref[4 * s + 1]);
+  const int act = (D.st[s].max_value != 0.f && rn > (double)D.tol) ? 1 : 0;
+  if (threadIdx.x == 0) D.stage[s] = act ? 0 : 1;
+  if (!act) return;
+  const float *G = D.g + s * n;
+  const int *mask = D.bins2 + s * n;
+  float *gs = D.grad + s * n;
+  double g2 = 0.0;
+  for (int j = threadIdx.x; j < D.n; j += blockDim.x) {
+    const float v = mask[j] ? G[j] : 0.f;
+    gs[j] = v;
+    g2 += (double)v * (double)v;
+  }
+  Team t{0, 1, nullptr, 0, nullptr};
+  double v[1] = {g2};
+  const int op[1] = {0};
+  team_reduce<double, 1>(t, v, op);
+  if (threadIdx.x == 0) D.ref[4 * s] = v[0];
+  mr_quant(t, D, gs, nullptr, D.n, D.vb1, D.sv1, s);
 }
 
-// After a refinement: a signal whose residual exceeds tol (the reference's
-// stopping test, now with the exact residual) resumes CGIHT from its refined
-// x with the exact residual, a fresh stopping history and a restarted
-// direction; the others stay finished.
-__global__ void __launch_bounds__(MR_THREADS) mr_mark_kernel(MrhsDesc D) {
+// x += mu g with the NIHT step mu = ||g_S||^2 / ||A g_S||^2 (active signals);
+// the refinement that follows keeps the k largest entries (hard thresholding
+// pursuit with an exact k-term threshold).
+__global__ void __launch_bounds__(MR_VT) mr_htp_step_kernel(MrhsDesc D) {
   const int s = blockIdx.x;
-  if (threadIdx.x != 0) return;
-  XState S = D.st[s];
-  const double rn = sqrt(D.ref[4 * s + 1]);
-  if (S.max_value == 0.f || !(rn > (double)D.tol)) {
-    D.stage[s] = 1;
-    return;
-  }
-  S.done = 0;
-  S.stop_reason = 0;
-  S.diff = 1.f;
-  S.cycle = 1;
-  S.fail = 0;
-  S.beta_to_zero = 1;
-  S.supp_change = 1;
-  for (int j = 0; j < 16; j++) S.evo[j] = 1.f;
-  D.st[s] = S;
-  D.tol_lp[s] = D.tol;
-  D.stage[s] = 0;
+  if (D.stage[s] != 0) return;
+  const long n = D.n, m = D.m;
+  const float *w = D.w1 + s * m, *G = D.g + s * n;
+  float *x = D.x + s * n;
+  double a2 = 0.0;
+  for (int i = threadIdx.x; i < D.m; i += blockDim.x) a2 += (double)w[i] * (double)w[i];
+  Team t{0, 1, nullptr, 0, nullptr};
+  double v[1] = {a2};
+  const int op[1] = {0};
+  team_reduce<double, 1>(t, v, op);
+  const float mu = v[0] > 0.0 ? (float)(D.ref[4 * s] / v[0]) : 0.f;
+  for (int j = threadIdx.x; j < D.n; j += blockDim.x) x[j] = fmaf(mu, G[j], x[j]);
 }
+
 
 // CG on A_S^T A_S d = A_S^T r, part 1 (after s = A^T r): gamma = ||s_S||^2,
 // p = s_S + beta p (beta = 0 on the first step); p as the next A operand.
-__global__ void __launch_bounds__(MR_THREADS) mr_ref_dir_kernel(MrhsDesc D, int first) {
+__global__ void __launch_bounds__(MR_VT) mr_ref_dir_kernel(MrhsDesc D, int first) {
   const int s = blockIdx.x;
   const long n = D.n;
   const float *G = D.g + s * n;
@@ -703,7 +744,7 @@ __global__ void __launch_bounds__(MR_THREADS) mr_ref_dir_kernel(MrhsDesc D, int 
 
 // CG part 2 (after q = A p): alpha = gamma / ||q||^2, d += alpha p,
 // r -= alpha q; r as the next A^T operand.
-__global__ void __launch_bounds__(MR_THREADS) mr_ref_step_kernel(MrhsDesc D) {
+__global__ void __launch_bounds__(MR_VT) mr_ref_step_kernel(MrhsDesc D) {
   const int s = blockIdx.x;
   const long n = D.n, m = D.m;
   const float *q = D.w1 + s * m, *p = D.gprev_thres + s * n;
@@ -723,7 +764,7 @@ __global__ void __launch_bounds__(MR_THREADS) mr_ref_step_kernel(MrhsDesc D) {
 }
 
 // x_S += d.
-__global__ void __launch_bounds__(MR_THREADS) mr_ref_update_kernel(MrhsDesc D) {
+__global__ void __launch_bounds__(MR_VT) mr_ref_update_kernel(MrhsDesc D) {
   const int s = blockIdx.x;
   const long n = D.n;
   float *x = D.x + s * n;
@@ -838,7 +879,7 @@ struct xcs_mrhs {
   int dev = 0;
   hipStream_t stream = nullptr;
   int m = 0, n = 0, k = 0, count = 0, prec = FMT_BF16;
-  int ref_outer = 3, ref_inner = 5;
+  int ref_outer = 2, ref_inner = 4;
   float c16 = 0.0078125f, c8 = 0.125f;
   MrhsDesc D{};
   float *A32 = nullptr, *cs = nullptr;
@@ -851,9 +892,9 @@ struct xcs_mrhs {
   double *part = nullptr;    // exact-residual partials (count x ceil(m / 256))
   double *rowpart = nullptr; // exact-residual row sums per slice (count x MR_SLICES x m)
   int nblk = 1;
-  int polish_max = 16;
+  int polish_max = 6;  // support-correction steps
   xcs_options opts{};
-  float phase_ms[6] = {0, 0, 0, 0, 0, 0};  // init, low-precision, bf16 continuation, refine, exact phase, refine
+  float phase_ms[6] = {0, 0, 0, 0, 0, 0};  // init, low precision, bf16 continuation, refinement, corrections, spare
   ~xcs_mrhs() {
     if (stream) (void)hipStreamSynchronize(stream);
     for (void *q : allocs) (void)hipFree(q);
@@ -926,6 +967,7 @@ static int mr_create(xcs_mrhs *h, int m, int n, int k, int count, const char *pr
     return set_err(XCS_EINVAL, "need 0 < k <= m, n > 0, count > 0");
   if (m % 128 || n % 512)
     return set_err(XCS_EINVAL, "many-rhs products need m %% 128 == 0 and n %% 512 == 0");
+  if (m > 4 * MR_THREADS * MR_MAXQ) return set_err(XCS_EINVAL, "many-rhs needs m <= %d", 4 * MR_THREADS * MR_MAXQ);
   XCS_TRY(use_device(o->gpu_number));
   HIP_TRY(hipGetDevice(&h->dev));
   HIP_TRY(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
@@ -1046,29 +1088,23 @@ static int mr_create(xcs_mrhs *h, int m, int n, int k, int count, const char *pr
   return XCS_OK;
 }
 
-static int mr_exact_resid(xcs_mrhs *h, int which) {
-  hipLaunchKernelGGL(mr_exact_resid_kernel, dim3(h->nblk, h->count, MR_SLICES), dim3(MR_THREADS), 0,
-                     h->stream, h->D, which, h->rowpart);
+static int mr_exact_resid(xcs_mrhs *h) {
+  hipLaunchKernelGGL(mr_exact_resid_kernel, dim3(MR_SLICES, h->count), dim3(MR_THREADS), 0,
+                     h->stream, h->D, h->rowpart);
   hipLaunchKernelGGL(mr_resid_sum_kernel, dim3(h->nblk, h->count), dim3(MR_THREADS), 0, h->stream,
-                     h->D, which, h->rowpart, h->part);
+                     h->D, h->rowpart, h->part);
   HIP_TRY(hipGetLastError());
   return XCS_OK;
 }
 
 // One lock-step CGIHT iteration over the active signals (format D.fmt).
-static int mr_iteration(xcs_mrhs *h, bool exact) {
+static int mr_iteration(xcs_mrhs *h) {
   MrhsDesc &D = h->D;
   const int f = D.fmt;
-  const dim3 gs(h->count), bs(MR_THREADS);
-  hipLaunchKernelGGL(mr_list_kernel, dim3(1), bs, 0, h->stream, D);
-  if (exact) {
-    hipLaunchKernelGGL(mr_ref_supp_kernel, gs, bs, 0, h->stream, D, 0);
-    XCS_TRY(mr_exact_resid(h, 1));
-    hipLaunchKernelGGL(mr_k1x_kernel, gs, bs, 0, h->stream, D, h->part, h->nblk);
-  } else {
-    XCS_TRY(mr_gemm(h, f, false, D.xb, D.ax, D.sx, D.nactive, D.active));
-    hipLaunchKernelGGL(mr_k1_kernel, gs, bs, 0, h->stream, D);
-  }
+  const dim3 gs(h->count), bs(MR_VT);
+  hipLaunchKernelGGL(mr_list_kernel, dim3(1), dim3(MR_THREADS), 0, h->stream, D);
+  XCS_TRY(mr_gemm(h, f, false, D.xb, D.ax, D.sx, D.nactive, D.active));
+  hipLaunchKernelGGL(mr_k1_kernel, gs, bs, 0, h->stream, D);
   XCS_TRY(mr_gemm(h, f, true, D.rb, D.g, D.sr, D.nactive, D.active));
   hipLaunchKernelGGL(mr_k2_kernel, gs, bs, 0, h->stream, D);
   XCS_TRY(mr_gemm(h, f, false, D.vb1, D.w1, D.sv1, D.nactive, D.active));
@@ -1080,16 +1116,16 @@ static int mr_iteration(xcs_mrhs *h, bool exact) {
 
 // Iterate until no signal is active (checked every 4 iterations) or the
 // iteration cap; returns the iterations issued.
-static int mr_phase(xcs_mrhs *h, bool exact, int *issued) {
+static int mr_phase(xcs_mrhs *h, int *issued) {
   MrhsDesc &D = h->D;
   int it = 0;
-  const int cap = exact ? h->polish_max : D.maxiter + 2;
+  const int cap = D.maxiter + 2;
   while (it < cap) {
     hipLaunchKernelGGL(mr_list_kernel, dim3(1), dim3(MR_THREADS), 0, h->stream, D);
     HIP_TRY(hipMemcpyAsync(h->h_nactive, D.nactive, sizeof(int), hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
     if (*h->h_nactive == 0) break;
-    for (int q = 0; q < 4 && it < cap; q++, it++) XCS_TRY(mr_iteration(h, exact));
+    for (int q = 0; q < 4 && it < cap; q++, it++) XCS_TRY(mr_iteration(h));
   }
   *issued = it;
   return XCS_OK;
@@ -1098,10 +1134,10 @@ static int mr_phase(xcs_mrhs *h, bool exact, int *issued) {
 static int mr_refine(xcs_mrhs *h) {
   MrhsDesc &D = h->D;
   D.fmt = FMT_BF16;
-  const dim3 gs(h->count), bs(MR_THREADS);
+  const dim3 gs(h->count), bs(MR_VT);
   hipLaunchKernelGGL(mr_ref_supp_kernel, gs, bs, 0, h->stream, D, 1);
   for (int o = 0; o < h->ref_outer; o++) {
-    XCS_TRY(mr_exact_resid(h, 0));
+    XCS_TRY(mr_exact_resid(h));
     hipLaunchKernelGGL(mr_ref_resid_kernel, gs, bs, 0, h->stream, D, h->part, h->nblk);
     XCS_TRY(mr_gemm(h, FMT_BF16, true, D.rb, D.g, nullptr, nullptr, nullptr));
     hipLaunchKernelGGL(mr_ref_dir_kernel, gs, bs, 0, h->stream, D, 1);
@@ -1115,7 +1151,7 @@ static int mr_refine(xcs_mrhs *h) {
     hipLaunchKernelGGL(mr_ref_update_kernel, gs, bs, 0, h->stream, D);
   }
   // final FP64 residual
-  XCS_TRY(mr_exact_resid(h, 0));
+  XCS_TRY(mr_exact_resid(h));
   hipLaunchKernelGGL(mr_ref_resid_kernel, gs, bs, 0, h->stream, D, h->part, h->nblk);
   HIP_TRY(hipGetLastError());
   return XCS_OK;
@@ -1126,7 +1162,7 @@ static int mr_solve(xcs_mrhs *h, float *ms) {
   hipEvent_t ev[7];
   for (int e = 0; e < 7; e++) HIP_TRY(hipEventCreate(&ev[e]));
   HIP_TRY(hipEventRecord(ev[0], h->stream));
-  const dim3 gs(h->count), bs(MR_THREADS);
+  const dim3 gs(h->count), bs(MR_VT);
   D.fmt = h->prec;
   const float c_lp = h->prec == FMT_FP8 ? h->c8 : h->c16;
   hipLaunchKernelGGL(mr_init0_kernel, gs, bs, 0, h->stream, D);
@@ -1139,24 +1175,34 @@ static int mr_solve(xcs_mrhs *h, float *ms) {
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(ev[1], h->stream));
   h->lp_iters[0] = h->lp_iters[1] = h->lp_iters[2] = 0;
-  XCS_TRY(mr_phase(h, false, &h->lp_iters[0]));
+  XCS_TRY(mr_phase(h, &h->lp_iters[0]));
   HIP_TRY(hipEventRecord(ev[2], h->stream));
   if (h->prec == FMT_FP8) {
     D.fmt = FMT_BF16;
     hipLaunchKernelGGL(mr_switch_kernel, gs, bs, 0, h->stream, D, h->c16);
     HIP_TRY(hipGetLastError());
-    XCS_TRY(mr_phase(h, false, &h->lp_iters[1]));
+    XCS_TRY(mr_phase(h, &h->lp_iters[1]));
   }
   HIP_TRY(hipEventRecord(ev[3], h->stream));
   XCS_TRY(mr_refine(h));
   HIP_TRY(hipEventRecord(ev[4], h->stream));
-  // signals above tol after the refinement: CGIHT with the exact residual
-  hipLaunchKernelGGL(mr_mark_kernel, gs, dim3(64), 0, h->stream, D);
-  HIP_TRY(hipGetLastError());
+  // signals above tol after the refinement: support corrections on the
+  // exact residual, each followed by the refinement
   D.fmt = FMT_BF16;
-  XCS_TRY(mr_phase(h, true, &h->lp_iters[2]));
+  for (int c = 0; c < h->polish_max; c++) {
+    XCS_TRY(mr_gemm(h, FMT_BF16, true, D.rb, D.g, nullptr, nullptr, nullptr));
+    hipLaunchKernelGGL(mr_htp_grad_kernel, gs, bs, 0, h->stream, D);
+    hipLaunchKernelGGL(mr_list_kernel, dim3(1), dim3(MR_THREADS), 0, h->stream, D);
+    HIP_TRY(hipMemcpyAsync(h->h_nactive, D.nactive, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    if (*h->h_nactive == 0) break;
+    XCS_TRY(mr_gemm(h, FMT_BF16, false, D.vb1, D.w1, nullptr, D.nactive, D.active));
+    hipLaunchKernelGGL(mr_htp_step_kernel, gs, bs, 0, h->stream, D);
+    HIP_TRY(hipGetLastError());
+    XCS_TRY(mr_refine(h));
+    h->lp_iters[2]++;
+  }
   HIP_TRY(hipEventRecord(ev[5], h->stream));
-  if (h->lp_iters[2] > 0) XCS_TRY(mr_refine(h));
   HIP_TRY(hipEventRecord(ev[6], h->stream));
   HIP_TRY(hipEventSynchronize(ev[6]));
   for (int e = 0; e < 6; e++) HIP_TRY(hipEventElapsedTime(&h->phase_ms[e], ev[e], ev[e + 1]));
@@ -1278,6 +1324,19 @@ int xcs_mrhs_fp32_solve(xcs_mrhs *h, int first, int cnt, int mode, float *ms_tot
     if (iterations) iterations[q] = pl.last.iter;
   }
   *ms_total = total;
+  return XCS_OK;
+}
+
+int xcs_mrhs_configure(xcs_mrhs *h, float c_bf16, float c_fp8, int exact_max, int ref_outer,
+                       int ref_inner) {
+  if (!h) return set_err(XCS_EINVAL, "NULL handle");
+  if (!(c_bf16 > 0.f) || !(c_fp8 > 0.f) || exact_max < 0 || ref_outer < 1 || ref_inner < 1)
+    return set_err(XCS_EINVAL, "need c_bf16, c_fp8 > 0, exact_max >= 0, ref_outer, ref_inner >= 1");
+  h->c16 = c_bf16;
+  h->c8 = c_fp8;
+  h->polish_max = exact_max;
+  h->ref_outer = ref_outer;
+  h->ref_inner = ref_inner;
   return XCS_OK;
 }
 

@@ -150,14 +150,16 @@ int xcs_batch_phase_ticks(xcs_batch *batch, unsigned long long *ticks32);
  * A ~ N(0, 1/m) from key (opts->seed, opts->instance); signal s is k-sparse with key
  * (opts->seed, opts->instance + 1 + s) and values by opts->vec_distribution; y = A x (FP32 A,
  * FP64 sums).  precision: "bf16" (A in BF16) or "fp8" (A in OCP e4m3 with per-column scales,
- * then BF16); both end with a support-restricted least-squares refinement (FP32 CG, BF16
- * products, FP64 residual from the FP32 A).  Needs m % 128 == 0, n % 512 == 0. */
+ * then BF16); both end with a least-squares refinement on the k largest entries (FP32 CG,
+ * BF16 products, FP64 residual from the FP32 A) and, for signals still above tol, hard
+ * thresholding pursuit corrections on the exact residual.  Needs m % 128 == 0,
+ * n % 512 == 0, m <= 8192. */
 typedef struct xcs_mrhs xcs_mrhs;
 int xcs_mrhs_create(int m, int n, int k, int count, const char *precision,
                     const xcs_options *opts, xcs_mrhs **handle);
 /* *ms = device time of the whole solve; lp_iterations[3] = lock-step iterations issued in the
- * FP8 (or BF16) phase, in the BF16 continuation (0 for "bf16"), and in the exact-residual
- * phase that resumes the signals still above tol after the first refinement. */
+ * FP8 (or BF16) phase and in the BF16 continuation (0 for "bf16"), and the support-correction
+ * steps taken for signals still above tol after the refinement. */
 int xcs_mrhs_solve(xcs_mrhs *handle, float *ms, int *lp_iterations);
 /* norms[3*s..]: relative l1/l2/linf error against the planted signal; iterations[s]: CGIHT
  * iterations; resid_rel[s] = ||y - A x|| / ||y|| in FP64 with the FP32 A. */
@@ -173,8 +175,13 @@ int xcs_mrhs_gemm_timed(xcs_mrhs *handle, int transpose, const char *fmt, int re
  * norms[3*q..] and iterations[q] per signal. */
 int xcs_mrhs_fp32_solve(xcs_mrhs *handle, int first, int cnt, int mode, float *ms_total,
                         float *norms, int *iterations);
+/* Solver knobs: low-precision stopping tolerance c * ||y|| for the BF16 and FP8 phases
+ * (defaults 2^-7, 2^-3), cap on support-correction steps (6), refinement outer and inner
+ * (CG) steps (2, 4). */
+int xcs_mrhs_configure(xcs_mrhs *handle, float c_bf16, float c_fp8, int corrections,
+                       int ref_outer, int ref_inner);
 /* Device ms of the last solve's phases: start, low-precision phase, BF16 continuation,
- * refinement, exact-residual phase, second refinement. */
+ * refinement, support corrections (with their refinements), 0. */
 int xcs_mrhs_phase_ms(xcs_mrhs *handle, float *ms6);
 void xcs_mrhs_destroy(xcs_mrhs *handle);
 

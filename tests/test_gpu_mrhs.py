@@ -78,7 +78,7 @@ def test_many_rhs_recovers(precision, oracle):
     ok = r["norms"][:, 2] < 1e-3
     print("%s: %.1f ms, lp iterations %s, success %d/%d, resid_rel median %.2e"
           % (precision, ms, lp, ok.sum(), count, np.median(r["resid_rel"])))
-    assert ok.mean() >= 0.95
+    assert ok.mean() >= 0.98
     # refined solutions sit at the FP32 floor of the residual
     assert np.median(r["resid_rel"][ok]) < 1e-5
     # against the FP64 oracle on the same problems (same generator, key (SEED, 5))
