@@ -16,9 +16,10 @@ import warnings
 import numpy as np
 
 from . import _capi
-from ._capi import Plan, Batch, ManyRHS, GagaError, gemm_tn, load  # noqa: F401
+from ._capi import Plan, Batch, ManyRHS, MLP, GagaError, gemm_tn, mlp_features, load  # noqa: F401
 
-__all__ = ["cs", "options", "Plan", "Batch", "ManyRHS", "gemm_tn", "GagaError", "load",
+__all__ = ["cs", "options", "Plan", "Batch", "ManyRHS", "MLP", "gemm_tn", "mlp_features",
+           "GagaError", "load",
            "ALGORITHMS", "ENSEMBLES"]
 
 ALGORITHMS = ("NIHT", "HTP", "CSMPSP", "CGIHT")

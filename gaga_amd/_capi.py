@@ -17,7 +17,10 @@ EXPORTS = (
     "xcs_batch_x", "xcs_batch_info", "xcs_batch_destroy", "xcs_batch_phase_ticks",
     "xcs_find_support_set", "xcs_apply", "xcs_generate_gen", "xcs_gemm_tn",
     "xcs_mrhs_create", "xcs_mrhs_solve", "xcs_mrhs_results", "xcs_mrhs_x",
-    "xcs_mrhs_gemm_timed", "xcs_mrhs_phase_ms", "xcs_mrhs_fp32_solve", "xcs_mrhs_configure", "xcs_mrhs_destroy",
+    "xcs_mrhs_gemm_timed", "xcs_mrhs_phase_ms", "xcs_mrhs_fp32_solve", "xcs_mrhs_configure",
+    "xcs_mrhs_samples", "xcs_mlp_create", "xcs_mlp_destroy", "xcs_mlp_param_count",
+    "xcs_mlp_get_params", "xcs_mlp_set_params", "xcs_mlp_grad", "xcs_mlp_scores",
+    "xcs_mlp_set_data", "xcs_mlp_train", "xcs_plan_solve_warm", "xcs_mlp_features", "xcs_mrhs_destroy",
     "xcs_last_error", "xcs_version",
 )
 
@@ -80,6 +83,19 @@ def load():
         L.xcs_mrhs_x.argtypes = [_P, _I, _P, _P, _P]
         L.xcs_mrhs_gemm_timed.argtypes = [_P, _I, _C, _I, _P, _P]
         L.xcs_mrhs_phase_ms.argtypes = [_P, _P]
+        L.xcs_mrhs_samples.argtypes = [_P, _I, ctypes.c_uint, _P, _P]
+        L.xcs_mlp_create.argtypes = [ctypes.c_uint, _P]
+        L.xcs_mlp_destroy.argtypes = [_P]
+        L.xcs_mlp_destroy.restype = None
+        L.xcs_mlp_param_count.argtypes = []
+        L.xcs_mlp_get_params.argtypes = [_P, _P]
+        L.xcs_mlp_set_params.argtypes = [_P, _P]
+        L.xcs_mlp_grad.argtypes = [_P, _P, _P, _I, ctypes.c_float, _P, _P]
+        L.xcs_mlp_scores.argtypes = [_P, _P, _I, _P]
+        L.xcs_mlp_set_data.argtypes = [_P, _P, _P, ctypes.c_long]
+        L.xcs_mlp_train.argtypes = [_P, _I, _I, ctypes.c_float, ctypes.c_float, _P, _P]
+        L.xcs_plan_solve_warm.argtypes = [_P, _P, _I, _P, _P, _P]
+        L.xcs_mlp_features.argtypes = [_P, _I, _I, _I, _P]
         L.xcs_mrhs_configure.argtypes = [_P, ctypes.c_float, ctypes.c_float, _I, _I, _I]
         L.xcs_mrhs_fp32_solve.argtypes = [_P, _I, _I, _I, _P, _P, _P]
         L.xcs_mrhs_destroy.argtypes = [_P]
@@ -235,6 +251,16 @@ class Plan:
         _ck(load().xcs_plan_bytes(self._h, ctypes.byref(b)))
         return b.value
 
+    def solve_warm(self, net=None, cg_steps=3):
+        """CGIHT from the network's predicted support (net None: the k largest
+        |A^T y|), plain CGIHT as fallback: (total ms, ms before the warm CGIHT,
+        fallback used)."""
+        ms, wms, fb = ctypes.c_float(0), ctypes.c_float(0), ctypes.c_int(0)
+        _ck(load().xcs_plan_solve_warm(self._h, net._h if net is not None else None,
+                                       int(cg_steps), ctypes.byref(ms), ctypes.byref(wms),
+                                       ctypes.byref(fb)))
+        return ms.value, wms.value, bool(fb.value)
+
     def apply_timed(self, transpose, reps):
         ms = ctypes.c_float(0)
         b = ctypes.c_double(0)
@@ -355,6 +381,14 @@ class Batch:
             pass
 
 
+def mlp_features(g, m, k):
+    """Warm-start features of one instance from g = A^T y (device kernel)."""
+    g = np.ascontiguousarray(g, np.float32)
+    X = np.zeros((g.size, 8), np.float32)
+    _ck(load().xcs_mlp_features(_p(g), g.size, int(m), int(k), _p(X)))
+    return X
+
+
 def gemm_tn(fmt, A, B):
     """C = q(A) q(B)^T on the MFMA GEMM; returns (C (M x N), A_q, B_q)."""
     A = np.ascontiguousarray(A, np.float32)
@@ -415,6 +449,13 @@ class ManyRHS:
         _ck(load().xcs_mrhs_configure(self._h, float(c_bf16), float(c_fp8), int(corrections),
                                       int(ref_outer), int(ref_inner)))
 
+    def samples(self, per, seed=1):
+        """Training samples of every signal: (X (count*per x 8), labels)."""
+        X = np.zeros((self.count * per, 8), np.float32)
+        y = np.zeros(self.count * per, np.float32)
+        _ck(load().xcs_mrhs_samples(self._h, int(per), int(seed), _p(X), _p(y)))
+        return X, y
+
     def phase_ms(self):
         out = np.zeros(6, np.float32)
         _ck(load().xcs_mrhs_phase_ms(self._h, _p(out)))
@@ -430,6 +471,64 @@ class ManyRHS:
     def close(self):
         if self._h:
             load().xcs_mrhs_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class MLP:
+    """The warm-start network (xcs_mlp_*): 8 features -> 512 -> 512 -> 1."""
+
+    def __init__(self, seed=1):
+        h = ctypes.c_void_p()
+        _ck(load().xcs_mlp_create(int(seed), ctypes.byref(h)))
+        self._h = h
+        self.nparams = load().xcs_mlp_param_count()
+
+    def params(self):
+        out = np.zeros(self.nparams, np.float32)
+        _ck(load().xcs_mlp_get_params(self._h, _p(out)))
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.nparams
+        _ck(load().xcs_mlp_set_params(self._h, _p(p)))
+
+    def grad(self, X, labels, pos_w=1.0):
+        X = np.ascontiguousarray(X, np.float32)
+        labels = np.ascontiguousarray(labels, np.float32)
+        g = np.zeros(self.nparams, np.float32)
+        loss = ctypes.c_float(0)
+        _ck(load().xcs_mlp_grad(self._h, _p(X), _p(labels), len(labels), float(pos_w), _p(g),
+                                ctypes.byref(loss)))
+        return g, loss.value
+
+    def scores(self, X):
+        X = np.ascontiguousarray(X, np.float32)
+        out = np.zeros(X.shape[0], np.float32)
+        _ck(load().xcs_mlp_scores(self._h, _p(X), X.shape[0], _p(out)))
+        return out
+
+    def set_data(self, X, labels):
+        self._X = np.ascontiguousarray(X, np.float32)
+        self._y = np.ascontiguousarray(labels, np.float32)
+        _ck(load().xcs_mlp_set_data(self._h, _p(self._X), _p(self._y), len(self._y)))
+
+    def train(self, steps, batch=8192, lr=1e-3, pos_w=1.0):
+        loss = ctypes.c_float(0)
+        ms = ctypes.c_float(0)
+        _ck(load().xcs_mlp_train(self._h, int(steps), int(batch), float(lr), float(pos_w),
+                                 ctypes.byref(loss), ctypes.byref(ms)))
+        return loss.value, ms.value
+
+    def close(self):
+        if self._h:
+            load().xcs_mlp_destroy(self._h)
             self._h = None
 
     def __del__(self):

@@ -57,6 +57,8 @@ struct XProblem {
   XG(float) trace_err;        // optional, maxiter + 2
   XG(int) trace_kbin;         // optional, maxiter + 2
   XG(unsigned long long) phase_ticks;  // optional (XCS_PHASE_TIMING builds): 32 counters
+  XG(const int) warm;         // optional CGIHT warm start: support mask (n), 1 = in
+  int warm_cg;                // CG steps of the least-squares start on that support
 };
 
 constexpr int MAXCH = 64;  // gen gather: max column chunks (partials rows)

@@ -1,0 +1,828 @@
+// Learned warm start for CGIHT: a per-coordinate MLP that scores how likely
+// each coordinate is in the support, from features of g = A^T y.
+//
+// No counterpart in the reference (which always starts CGIHT from the
+// thresholded A^T y, GAGA/gaga/algorithms.cu:3752-3790).  Network: F = 8
+// features -> 512 -> 512 -> 1 logit, ReLU between layers; weights kept in FP32
+// (Adam), layer 2 runs on the BF16 MFMA GEMM with FP32 accumulation.
+//
+// Features of coordinate j (a = |g_j|; t_q is the q-th largest |g|):
+//   a/t_k, a/t_2k, a/t_{ceil(k/2)}, a/max|g|, a/rms(g), delta = m/n, rho = k/m,
+//   min(log2(1 + a/rms(g)), 8) / 8.
+//
+// Training (on the GPU): binary cross-entropy on "j is in the recovered
+// support", forward and backward by the kernels below, Adam.  Reductions over
+// the batch are done in fixed order (deterministic gradients).
+
+namespace xcs {
+
+constexpr int MLP_F = 8;
+constexpr int MLP_H = 512;
+// parameter layout (floats): W1[H][F], b1[H], W2[H][H] (out x in), b2[H], W3[H], b3
+constexpr long MLP_OW1 = 0, MLP_OB1 = MLP_H * MLP_F, MLP_OW2 = MLP_OB1 + MLP_H,
+               MLP_OB2 = MLP_OW2 + (long)MLP_H * MLP_H, MLP_OW3 = MLP_OB2 + MLP_H,
+               MLP_OB3 = MLP_OW3 + MLP_H, MLP_NP = MLP_OB3 + 1;
+
+__device__ __forceinline__ float bf16_to_f(uint16_t b) {
+  const uint32_t u = (uint32_t)b << 16;
+  float f;
+  __builtin_memcpy(&f, &u, 4);
+  return f;
+}
+
+// ------------------------------------------------------------- features ----
+// k-th largest |v| over n entries (one workgroup; bitwise search on the IEEE
+// bits of |v|).  k >= 1, k <= n.  Uniform result.
+__device__ float wg_kth_largest_abs(const float *v, int n, int k, int *scratch16) {
+  auto count_ge = [&](uint32_t thr) {
+    int c = 0;
+    for (int j = threadIdx.x; j < n; j += blockDim.x)
+      c += ((__float_as_uint(v[j]) & 0x7FFFFFFFu) >= thr) ? 1 : 0;
+    c = wave_sum_i(c);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) scratch16[threadIdx.x >> 6] = c;
+    __syncthreads();
+    int t = 0;
+    for (int w = 0; w < (int)blockDim.x / 64; w++) t += scratch16[w];
+    return t;
+  };
+  uint32_t t = 0;
+  for (int bit = 30; bit >= 0; bit--) {
+    const uint32_t cand = t | (1u << bit);
+    if (count_ge(cand) >= k) t = cand;
+  }
+  return __uint_as_float(t);
+}
+
+// Features of every coordinate of one instance (one workgroup): X[j*F + f].
+__device__ void mlp_features(const float *g, int n, int m, int k, float *X) {
+  __shared__ int scr[MAX_WAVES];
+  __shared__ double red[2][MAX_WAVES];
+  const int kk = max(1, min(k, n));
+  const float tk = wg_kth_largest_abs(g, n, kk, scr);
+  const float t2k = wg_kth_largest_abs(g, n, min(n, 2 * kk), scr);
+  const float th = wg_kth_largest_abs(g, n, max(1, (kk + 1) / 2), scr);
+  double mx = 0.0, s2 = 0.0;
+  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+    const double a = fabs((double)g[j]);
+    mx = fmax(mx, a);
+    s2 += a * a;
+  }
+  mx = wave_max(mx);
+  s2 = wave_sum(s2);
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = mx;
+    red[1][threadIdx.x >> 6] = s2;
+  }
+  __syncthreads();
+  double M = 0.0, Q = 0.0;
+  for (int w = 0; w < (int)blockDim.x / 64; w++) {
+    M = fmax(M, red[0][w]);
+    Q += red[1][w];
+  }
+  const float rms = (float)sqrt(Q / n);
+  const float fm = (float)M;
+  auto inv = [](float d) { return d > 0.f ? 1.0f / d : 0.f; };
+  const float itk = inv(tk), it2k = inv(t2k), ith = inv(th), imx = inv(fm), irms = inv(rms);
+  const float delta = (float)m / (float)n, rho = (float)k / (float)m;
+  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+    const float a = fabsf(g[j]);
+    float *x = X + (long)j * MLP_F;
+    x[0] = a * itk;
+    x[1] = a * it2k;
+    x[2] = a * ith;
+    x[3] = a * imx;
+    x[4] = a * irms;
+    x[5] = delta;
+    x[6] = rho;
+    x[7] = fminf(log2f(1.0f + a * irms), 8.0f) * 0.125f;
+  }
+  __syncthreads();
+}
+
+// ------------------------------------------------------------- forward -----
+// h1[c][u] = bf16(relu(sum_f X[c][f] W1[u][f] + b1[u])), one thread per (c, u).
+__global__ void mlp_l1_kernel(const float *X, int B, const float *p, uint16_t *h1b) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)B * MLP_H) return;
+  const int c = (int)(e / MLP_H), u = (int)(e % MLP_H);
+  const float *x = X + (long)c * MLP_F, *w = p + MLP_OW1 + (long)u * MLP_F;
+  float z = p[MLP_OB1 + u];
+#pragma unroll
+  for (int f = 0; f < MLP_F; f++) z = fmaf(x[f], w[f], z);
+  h1b[e] = f2bf16_rne(fmaxf(z, 0.f));
+}
+
+// Logit of each coordinate (one wave per coordinate) from z2 = W2 h1 (GEMM):
+// a2 = relu(z2 + b2), logit = W3 . a2 + b3.  With labels (training): the
+// weighted BCE loss, dlogit = w (sigmoid - y) / B, dz2 = dlogit W3 [a2 > 0]
+// (FP32 and BF16 copies) and a2 kept for the weight gradients.
+__global__ void __launch_bounds__(256) mlp_l3_kernel(const float *z2, int B, const float *p,
+                                                     const float *labels, float pos_w,
+                                                     float *logit, float *loss, float *a2,
+                                                     float *dlogit, float *dz2,
+                                                     uint16_t *dz2b) {
+  const int lane = threadIdx.x & 63;
+  const long c = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (c >= B) return;
+  const float *z = z2 + c * MLP_H;
+  float acc = 0.f;
+  float av[MLP_H / 64];
+#pragma unroll
+  for (int q = 0; q < MLP_H / 64; q++) {
+    const int u = lane + 64 * q;
+    av[q] = fmaxf(z[u] + p[MLP_OB2 + u], 0.f);
+    acc = fmaf(av[q], p[MLP_OW3 + u], acc);
+  }
+  acc = wave_sum(acc);
+  const float lg = acc + p[MLP_OB3];
+  if (lane == 0) logit[c] = lg;
+  if (!labels) return;
+  const float y = labels[c];
+  const float w = y > 0.5f ? pos_w : 1.0f;
+  const float sig = 1.0f / (1.0f + expf(-lg));
+  const float dl = w * (sig - y) / (float)B;
+  if (lane == 0) {
+    loss[c] = w * (fmaxf(lg, 0.f) - lg * y + log1pf(expf(-fabsf(lg))));
+    dlogit[c] = dl;
+  }
+#pragma unroll
+  for (int q = 0; q < MLP_H / 64; q++) {
+    const int u = lane + 64 * q;
+    a2[c * MLP_H + u] = av[q];
+    const float d = av[q] > 0.f ? dl * p[MLP_OW3 + u] : 0.f;
+    dz2[c * MLP_H + u] = d;
+    dz2b[c * MLP_H + u] = f2bf16_rne(d);
+  }
+}
+
+// ------------------------------------------------------------ backward -----
+// Column sums over the batch in fixed order: out[u] = sum_c a[c*ld + u] * (w ? w[c] : 1),
+// for u < cols.  Stage 1: partials per (chunk of rows); stage 2 sums chunks in order.
+constexpr int MLP_RCHUNK = 256;
+__global__ void mlp_colsum_partial(const float *a, int B, int cols, long ld, const float *w,
+                                   float *part) {
+  const int u = blockIdx.x * blockDim.x + threadIdx.x, ch = blockIdx.y;
+  if (u >= cols) return;
+  const int c0 = ch * MLP_RCHUNK, c1 = min(B, c0 + MLP_RCHUNK);
+  float s = 0.f;
+  for (int c = c0; c < c1; c++) s = fmaf(a[(long)c * ld + u], w ? w[c] : 1.0f, s);
+  part[(long)ch * cols + u] = s;
+}
+__global__ void mlp_colsum_final(const float *part, int nch, int cols, float *out) {
+  const int u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= cols) return;
+  float s = 0.f;
+  for (int ch = 0; ch < nch; ch++) s += part[(long)ch * cols + u];
+  out[u] = s;
+}
+
+// dz1 = dh1 [h1 > 0]; in place on dh1.
+__global__ void mlp_relu_back_kernel(float *dh1, const uint16_t *h1b, long cnt) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < cnt && !(bf16_to_f(h1b[e]) > 0.f)) dh1[e] = 0.f;
+}
+
+// dW1[u][f] = sum_c dz1[c][u] X[c][f] (one thread per (u, f), fixed order).
+__global__ void mlp_dw1_partial(const float *dz1, const float *X, int B, float *part) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x, ch = blockIdx.y;
+  if (e >= MLP_H * MLP_F) return;
+  const int u = e / MLP_F, f = e % MLP_F;
+  const int c0 = ch * MLP_RCHUNK, c1 = min(B, c0 + MLP_RCHUNK);
+  float s = 0.f;
+  for (int c = c0; c < c1; c++) s = fmaf(dz1[(long)c * MLP_H + u], X[(long)c * MLP_F + f], s);
+  part[(long)ch * MLP_H * MLP_F + e] = s;
+}
+
+// FP32 tiled GEMM, C[i][j] = sum_k A(i, k) B(j, k) with strided operands
+// (B may be BF16), 64 x 64 tile per 256-thread block, 4 x 4 per thread, K in
+// steps of 16 through LDS; blockIdx.z = K split (slice z of C at z * c_split).
+template <bool BB16>
+__global__ void __launch_bounds__(256) f32_gemm_kernel(int M, int N, int K, const float *A,
+                                                       long sai, long sak, const void *Bv,
+                                                       long sbj, long sbk, float *C, long sci,
+                                                       long scj, int ksplit, long c_split) {
+  __shared__ float As[16][64 + 4], Bs[16][64 + 4];
+  const int i0 = blockIdx.x * 64, j0 = blockIdx.y * 64;
+  const int kc = (K + ksplit - 1) / ksplit, ka = blockIdx.z * kc, kb = min(K, ka + kc);
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  float acc[4][4] = {};
+  // consecutive threads walk the operand's contiguous index (K or the row)
+  const bool a_kc = sak == 1, b_kc = sbk == 1;
+  for (int k0 = ka; k0 < kb; k0 += 16) {
+    for (int e = threadIdx.x; e < 16 * 64; e += 256) {
+      const int kk = a_kc ? (e & 15) : (e >> 6), r = a_kc ? (e >> 4) : (e & 63);
+      const int k = k0 + kk, i = i0 + r;
+      As[kk][r] = (k < kb && i < M) ? A[(long)i * sai + (long)k * sak] : 0.f;
+    }
+    for (int e = threadIdx.x; e < 16 * 64; e += 256) {
+      const int kk = b_kc ? (e & 15) : (e >> 6), r = b_kc ? (e >> 4) : (e & 63);
+      const int k = k0 + kk, j = j0 + r;
+      float bv = 0.f;
+      if (k < kb && j < N) {
+        const long off = (long)j * sbj + (long)k * sbk;
+        bv = BB16 ? bf16_to_f(((const uint16_t *)Bv)[off]) : ((const float *)Bv)[off];
+      }
+      Bs[kk][r] = bv;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < 16; kk++) {
+      float a[4], b[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        a[q] = As[kk][ty * 4 + q];
+        b[q] = Bs[kk][tx * 4 + q];
+      }
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) acc[q][r] = fmaf(a[q], b[r], acc[q][r]);
+    }
+    __syncthreads();
+  }
+  float *Cz = C + (long)blockIdx.z * c_split;
+#pragma unroll
+  for (int q = 0; q < 4; q++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int i = i0 + ty * 4 + q, j = j0 + tx * 4 + r;
+      if (i < M && j < N) Cz[(long)i * sci + (long)j * scj] = acc[q][r];
+    }
+}
+
+// Sum of K-split slices in order: out[e] = sum_z part[z * stride + e].
+__global__ void mlp_split_sum(const float *part, int nz, long stride, long cnt, float *out) {
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < cnt;
+       e += (long)gridDim.x * blockDim.x) {
+    float s = 0.f;
+    for (int z = 0; z < nz; z++) s += part[z * stride + e];
+    out[e] = s;
+  }
+}
+
+// Adam (Kingma & Ba 2015) on the FP32 parameters; refreshes the BF16 copies
+// of W2 ([out][in] and transposed).
+__global__ void mlp_adam_kernel(float *p, float *m1, float *m2, const float *g, long np,
+                                float lr, float b1, float b2, float eps, float c1, float c2,
+                                uint16_t *w2b, uint16_t *w2tb) {
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < np;
+       e += (long)gridDim.x * blockDim.x) {
+    const float gi = g[e];
+    const float a = b1 * m1[e] + (1.f - b1) * gi;
+    const float b = b2 * m2[e] + (1.f - b2) * gi * gi;
+    m1[e] = a;
+    m2[e] = b;
+    const float v = p[e] - lr * (a / c1) / (sqrtf(b / c2) + eps);
+    p[e] = v;
+    if (e >= MLP_OW2 && e < MLP_OB2) {
+      const long q = e - MLP_OW2;
+      const int o = (int)(q / MLP_H), i = (int)(q % MLP_H);
+      const uint16_t h = f2bf16_rne(v);
+      w2b[q] = h;
+      w2tb[(long)i * MLP_H + o] = h;
+    }
+  }
+}
+
+// --------------------------------------------------- instance-level kernels --
+// Features of every coordinate of each many-RHS signal from g = A^T y (FP32).
+__global__ void __launch_bounds__(1024) mlp_features_kernel(const float *G, int n, int m, int k,
+                                                            float *X) {
+  const int s = blockIdx.x;
+  mlp_features(G + (long)s * n, n, m, k, X + (long)s * n * MLP_F);
+}
+
+// Training samples of each signal: half drawn from the 2k largest |g|
+// (candidates near the support), half uniform over all n; label = x_j != 0
+// (the recovered support).  Philox key (seed, signal).
+__global__ void __launch_bounds__(1024) mlp_samples_kernel(const float *G, const float *Xf,
+                                                           const float *xrec, int n, int k,
+                                                           int per, uint32_t seed, int *pool,
+                                                           float *Xo, float *Lo) {
+  __shared__ int scr[MAX_WAVES];
+  __shared__ int base;
+  const int s = blockIdx.x;
+  const float *g = G + (long)s * n;
+  const int kk = max(1, min(k, n));
+  const float t2k = wg_kth_largest_abs(g, n, min(n, 2 * kk), scr);
+  int *pl = pool + (long)s * n;
+  if (threadIdx.x == 0) base = 0;
+  __syncthreads();
+  for (int j0 = 0; j0 < n; j0 += blockDim.x) {
+    const int j = j0 + threadIdx.x;
+    const int in = (j < n && fabsf(g[j]) >= t2k) ? 1 : 0;
+    int tot;
+    const int pos = wg_exclusive_scan(in, &tot);
+    if (in) pl[base + pos] = j;
+    __syncthreads();
+    if (threadIdx.x == 0) base += tot;
+    __syncthreads();
+  }
+  const int psz = max(1, base);
+  for (int q = threadIdx.x; q < per; q += blockDim.x) {
+    uint32_t o[4];
+    philox4x32_10((uint32_t)q, (uint32_t)s, 7u, 0u, seed, 0x5A17u, o);
+    const int j = (q < per / 2) ? pl[bounded_u32(o[0], (uint32_t)psz)] : (int)bounded_u32(o[1], (uint32_t)n);
+    const long r = (long)s * per + q;
+#pragma unroll
+    for (int f = 0; f < MLP_F; f++) Xo[r * MLP_F + f] = Xf[((long)s * n + j) * MLP_F + f];
+    Lo[r] = xrec[(long)s * n + j] != 0.f ? 1.f : 0.f;
+  }
+}
+
+// Warm-start mask: the k coordinates with the largest scores (ties in index
+// order), one workgroup.
+__global__ void __launch_bounds__(1024) mlp_topk_mask_kernel(const float *score, int n, int k,
+                                                             int *mask) {
+  __shared__ int scr[MAX_WAVES];
+  __shared__ int base;
+  // order-preserving map of floats to unsigned keys
+  auto key = [](float v) {
+    const uint32_t u = __float_as_uint(v);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+  };
+  auto count_ge = [&](uint32_t thr) {
+    int c = 0;
+    for (int j = threadIdx.x; j < n; j += blockDim.x) c += key(score[j]) >= thr ? 1 : 0;
+    c = wave_sum_i(c);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) scr[threadIdx.x >> 6] = c;
+    __syncthreads();
+    int t = 0;
+    for (int w = 0; w < (int)blockDim.x / 64; w++) t += scr[w];
+    return t;
+  };
+  const int kk = max(1, min(k, n));
+  uint32_t t = 0;
+  for (int bit = 31; bit >= 0; bit--) {
+    const uint32_t cand = t | (1u << bit);
+    if (count_ge(cand) >= kk) t = cand;
+  }
+  const int quota = kk - (t == 0xFFFFFFFFu ? 0 : count_ge(t + 1u));
+  if (threadIdx.x == 0) base = 0;
+  __syncthreads();
+  for (int j0 = 0; j0 < n; j0 += blockDim.x) {
+    const int j = j0 + threadIdx.x;
+    const uint32_t kv = j < n ? key(score[j]) : 0u;
+    const int eq = (j < n && kv == t) ? 1 : 0;
+    int tot;
+    const int pos = wg_exclusive_scan(eq, &tot);
+    if (j < n) mask[j] = (kv > t) || (eq && base + pos < quota) ? 1 : 0;
+    __syncthreads();
+    if (threadIdx.x == 0) base += tot;
+    __syncthreads();
+  }
+}
+
+__global__ void mlp_abs_kernel(const float *g, int n, float *out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) out[j] = fabsf(g[j]);
+}
+
+// Features of one instance from its g (single workgroup).
+__global__ void __launch_bounds__(1024) mlp_features1_kernel(const float *g, int n, int m, int k,
+                                                             float *X) {
+  mlp_features(g, n, m, k, X);
+}
+
+}  // namespace xcs
+
+// ================================================================== host ====
+// (included by xcs_api.hip after the plan and many-RHS code)
+
+struct xcs_mlp {
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  float *p = nullptr, *m1 = nullptr, *m2 = nullptr, *g = nullptr;
+  uint16_t *w2b = nullptr, *w2tb = nullptr;
+  long step = 0;
+  // batch buffers (capacity cap coordinates)
+  long cap = 0;
+  float *X = nullptr, *lab = nullptr, *z2 = nullptr, *a2 = nullptr, *dz2 = nullptr, *dh1 = nullptr;
+  float *logit = nullptr, *loss = nullptr, *dlogit = nullptr, *part = nullptr, *gpart = nullptr;
+  uint16_t *h1b = nullptr, *dz2b = nullptr;
+  std::vector<void *> batch_allocs;
+  // training set
+  float *dX = nullptr, *dL = nullptr;
+  long ndata = 0;
+  std::vector<void *> allocs;
+  ~xcs_mlp() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (void *q : batch_allocs) (void)hipFree(q);
+    for (void *q : allocs) (void)hipFree(q);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+  template <class T>
+  int alloc_in(std::vector<void *> &list, T **q, size_t cnt) {
+    void *r = nullptr;
+    const size_t bytes = std::max<size_t>(cnt, 1) * sizeof(T);
+    hipError_t e = hipMalloc(&r, bytes);
+    if (e != hipSuccess)
+      return set_err(XCS_ENOMEM, "hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(e));
+    list.push_back(r);
+    e = hipMemsetAsync(r, 0, bytes, stream);
+    if (e != hipSuccess) return set_err(XCS_EHIP, "hipMemsetAsync: %s", hipGetErrorString(e));
+    *q = (T *)r;
+    return XCS_OK;
+  }
+};
+
+constexpr int MLP_KSPLIT = 16;  // K splits of the dW2 product (K = batch)
+
+// Batch buffers for up to B coordinates (grown on demand).
+static int mlp_reserve(xcs_mlp *net, long B) {
+  if (B <= net->cap) return XCS_OK;
+  HIP_TRY(hipStreamSynchronize(net->stream));
+  for (void *q : net->batch_allocs) (void)hipFree(q);
+  net->batch_allocs.clear();
+  const long H = MLP_H;
+  auto &L = net->batch_allocs;
+  XCS_TRY(net->alloc_in(L, &net->X, (size_t)B * MLP_F));
+  XCS_TRY(net->alloc_in(L, &net->lab, (size_t)B));
+  XCS_TRY(net->alloc_in(L, &net->h1b, (size_t)B * H));
+  XCS_TRY(net->alloc_in(L, &net->z2, (size_t)B * H));
+  XCS_TRY(net->alloc_in(L, &net->a2, (size_t)B * H));
+  XCS_TRY(net->alloc_in(L, &net->dz2, (size_t)B * H));
+  XCS_TRY(net->alloc_in(L, &net->dz2b, (size_t)B * H));
+  XCS_TRY(net->alloc_in(L, &net->dh1, (size_t)B * H));
+  XCS_TRY(net->alloc_in(L, &net->logit, (size_t)B));
+  XCS_TRY(net->alloc_in(L, &net->loss, (size_t)B));
+  XCS_TRY(net->alloc_in(L, &net->dlogit, (size_t)B));
+  const long nch = (B + MLP_RCHUNK - 1) / MLP_RCHUNK;
+  XCS_TRY(net->alloc_in(L, &net->part, (size_t)nch * H * MLP_F + (size_t)nch * H + 64));
+  XCS_TRY(net->alloc_in(L, &net->gpart, (size_t)MLP_KSPLIT * H * H));
+  net->cap = B;
+  return XCS_OK;
+}
+
+// Forward over B coordinates in net->X; logits in net->logit.  With labels
+// in net->lab: backward into net->g (and per-coordinate losses).
+static int mlp_pass(xcs_mlp *net, int B, bool train, float pos_w) {
+  const long H = MLP_H;
+  hipStream_t st = net->stream;
+  hipLaunchKernelGGL(mlp_l1_kernel, dim3((unsigned)((B * H + 255) / 256)), dim3(256), 0, st, net->X,
+                     B, net->p, net->h1b);
+  GemmArgs ga{};
+  ga.A = (const uint8_t *)net->w2b;
+  ga.lda_bytes = 2 * H;
+  ga.B = (const uint8_t *)net->h1b;
+  ga.ldb_bytes = 2 * H;
+  ga.C = net->z2;
+  ga.ldc = H;
+  ga.M = (int)H;
+  ga.K = (int)H;
+  ga.N = B;
+  ga.ksplit = 1;
+  hipLaunchKernelGGL(gemm_tn_kernel<FMT_BF16>, dim3(H / GT_M, (B + GT_N - 1) / GT_N, 1),
+                     dim3(GT_THREADS), 0, st, ga);
+  hipLaunchKernelGGL(mlp_l3_kernel, dim3((unsigned)((B * 64L + 255) / 256)), dim3(256), 0, st,
+                     net->z2, B, net->p, train ? net->lab : nullptr, pos_w, net->logit, net->loss,
+                     net->a2, net->dlogit, net->dz2, net->dz2b);
+  HIP_TRY(hipGetLastError());
+  if (!train) return XCS_OK;
+  const int nch = (B + MLP_RCHUNK - 1) / MLP_RCHUNK;
+  float *gp = net->g;
+  // dW3 = sum_c dlogit_c a2[c], db3 = sum_c dlogit_c, db2 = sum_c dz2[c]
+  hipLaunchKernelGGL(mlp_colsum_partial, dim3(H / 256, nch), dim3(256), 0, st, net->a2, B, (int)H,
+                     H, net->dlogit, net->part);
+  hipLaunchKernelGGL(mlp_colsum_final, dim3(H / 256), dim3(256), 0, st, net->part, nch, (int)H,
+                     gp + MLP_OW3);
+  hipLaunchKernelGGL(mlp_colsum_partial, dim3(1, nch), dim3(64), 0, st, net->dlogit, B, 1, 1L,
+                     (const float *)nullptr, net->part);
+  hipLaunchKernelGGL(mlp_colsum_final, dim3(1), dim3(64), 0, st, net->part, nch, 1, gp + MLP_OB3);
+  hipLaunchKernelGGL(mlp_colsum_partial, dim3(H / 256, nch), dim3(256), 0, st, net->dz2, B, (int)H,
+                     H, (const float *)nullptr, net->part);
+  hipLaunchKernelGGL(mlp_colsum_final, dim3(H / 256), dim3(256), 0, st, net->part, nch, (int)H,
+                     gp + MLP_OB2);
+  // dW2[o][i] = sum_c dz2[c][o] h1[c][i] (FP32, K split over the batch)
+  hipLaunchKernelGGL(f32_gemm_kernel<true>, dim3(H / 64, H / 64, MLP_KSPLIT), dim3(256), 0, st,
+                     (int)H, (int)H, B, net->dz2, 1L, H, (const void *)net->h1b, 1L, H, net->gpart,
+                     H, 1L, MLP_KSPLIT, H * H);
+  hipLaunchKernelGGL(mlp_split_sum, dim3(1024), dim3(256), 0, st, net->gpart, MLP_KSPLIT, H * H,
+                     H * H, gp + MLP_OW2);
+  // dh1 = dz2 W2 (MFMA: A = W2^T, B = dz2 in BF16), then dz1 = dh1 [h1 > 0]
+  GemmArgs gb{};
+  gb.A = (const uint8_t *)net->w2tb;
+  gb.lda_bytes = 2 * H;
+  gb.B = (const uint8_t *)net->dz2b;
+  gb.ldb_bytes = 2 * H;
+  gb.C = net->dh1;
+  gb.ldc = H;
+  gb.M = (int)H;
+  gb.K = (int)H;
+  gb.N = B;
+  gb.ksplit = 1;
+  hipLaunchKernelGGL(gemm_tn_kernel<FMT_BF16>, dim3(H / GT_M, (B + GT_N - 1) / GT_N, 1),
+                     dim3(GT_THREADS), 0, st, gb);
+  hipLaunchKernelGGL(mlp_relu_back_kernel, dim3((unsigned)((B * H + 255) / 256)), dim3(256), 0, st,
+                     net->dh1, net->h1b, (long)B * H);
+  // dW1, db1
+  hipLaunchKernelGGL(mlp_dw1_partial, dim3((MLP_H * MLP_F + 255) / 256, nch), dim3(256), 0, st,
+                     net->dh1, net->X, B, net->part);
+  hipLaunchKernelGGL(mlp_colsum_final, dim3((MLP_H * MLP_F + 255) / 256), dim3(256), 0, st,
+                     net->part, nch, MLP_H * MLP_F, gp + MLP_OW1);
+  hipLaunchKernelGGL(mlp_colsum_partial, dim3(H / 256, nch), dim3(256), 0, st, net->dh1, B, (int)H,
+                     H, (const float *)nullptr, net->part);
+  hipLaunchKernelGGL(mlp_colsum_final, dim3(H / 256), dim3(256), 0, st, net->part, nch, (int)H,
+                     gp + MLP_OB1);
+  HIP_TRY(hipGetLastError());
+  return XCS_OK;
+}
+
+static int mlp_refresh_bf16(xcs_mlp *net) {
+  // Adam with lr = 0 and zero gradients only rewrites the BF16 copies
+  hipLaunchKernelGGL(mlp_adam_kernel, dim3(1024), dim3(256), 0, net->stream, net->p, net->m1,
+                     net->m2, net->g, MLP_NP, 0.0f, 0.0f, 0.0f, 1.0f, 1.0f, 1.0f, net->w2b,
+                     net->w2tb);
+  HIP_TRY(hipGetLastError());
+  return XCS_OK;
+}
+
+extern "C" {
+
+int xcs_mlp_create(unsigned int seed, xcs_mlp **out) {
+  if (!out) return set_err(XCS_EINVAL, "NULL argument");
+  *out = nullptr;
+  xcs_mlp *net = new xcs_mlp();
+  auto fail = [&](int r) {
+    delete net;
+    return r;
+  };
+  int r = use_device(0);
+  if (r) return fail(r);
+  if (hipGetDevice(&net->dev) != hipSuccess || hipStreamCreateWithFlags(&net->stream, hipStreamNonBlocking) != hipSuccess)
+    return fail(set_err(XCS_EHIP, "stream creation failed"));
+  if ((r = net->alloc_in(net->allocs, &net->p, MLP_NP)) || (r = net->alloc_in(net->allocs, &net->m1, MLP_NP)) ||
+      (r = net->alloc_in(net->allocs, &net->m2, MLP_NP)) || (r = net->alloc_in(net->allocs, &net->g, MLP_NP)) ||
+      (r = net->alloc_in(net->allocs, &net->w2b, (size_t)MLP_H * MLP_H)) ||
+      (r = net->alloc_in(net->allocs, &net->w2tb, (size_t)MLP_H * MLP_H)))
+    return fail(r);
+  // He initialisation from Philox (key (seed, 0x4D4C50)), biases 0
+  std::vector<float> p((size_t)MLP_NP, 0.f);
+  auto gauss = [&](long idx) {
+    uint32_t o[4];
+    philox4x32_10((uint32_t)idx, 0u, 9u, 0u, seed, 0x4D4C50u, o);
+    return (float)(sqrt(-2.0 * log(u01d(o[0]))) * cos(6.283185307179586 * u01d(o[1])));
+  };
+  for (long e = 0; e < (long)MLP_H * MLP_F; e++) p[MLP_OW1 + e] = gauss(MLP_OW1 + e) * sqrtf(2.0f / MLP_F);
+  for (long e = 0; e < (long)MLP_H * MLP_H; e++) p[MLP_OW2 + e] = gauss(MLP_OW2 + e) * sqrtf(2.0f / MLP_H);
+  for (long e = 0; e < MLP_H; e++) p[MLP_OW3 + e] = gauss(MLP_OW3 + e) * sqrtf(1.0f / MLP_H);
+  if (hipMemcpyAsync(net->p, p.data(), 4 * p.size(), hipMemcpyHostToDevice, net->stream) != hipSuccess)
+    return fail(set_err(XCS_EHIP, "parameter upload failed"));
+  if ((r = mlp_refresh_bf16(net))) return fail(r);
+  if (hipStreamSynchronize(net->stream) != hipSuccess) return fail(set_err(XCS_EHIP, "sync failed"));
+  *out = net;
+  return XCS_OK;
+}
+
+void xcs_mlp_destroy(xcs_mlp *net) { delete net; }
+
+int xcs_mlp_param_count(void) { return (int)MLP_NP; }
+
+int xcs_mlp_get_params(xcs_mlp *net, float *out) {
+  if (!net || !out) return set_err(XCS_EINVAL, "NULL argument");
+  HIP_TRY(hipSetDevice(net->dev));
+  HIP_TRY(hipStreamSynchronize(net->stream));
+  HIP_TRY(hipMemcpy(out, net->p, 4 * (size_t)MLP_NP, hipMemcpyDeviceToHost));
+  return XCS_OK;
+}
+
+int xcs_mlp_set_params(xcs_mlp *net, const float *in) {
+  if (!net || !in) return set_err(XCS_EINVAL, "NULL argument");
+  HIP_TRY(hipSetDevice(net->dev));
+  HIP_TRY(hipMemcpyAsync(net->p, in, 4 * (size_t)MLP_NP, hipMemcpyHostToDevice, net->stream));
+  XCS_TRY(mlp_refresh_bf16(net));
+  HIP_TRY(hipStreamSynchronize(net->stream));
+  return XCS_OK;
+}
+
+// Forward and backward on a host batch: gradients (xcs_mlp_param_count floats,
+// parameter layout W1, b1, W2, b2, W3, b3) and the mean loss.
+int xcs_mlp_grad(xcs_mlp *net, const float *X, const float *labels, int B, float pos_w,
+                 float *grad_out, float *loss_out) {
+  if (!net || !X || !labels || B <= 0) return set_err(XCS_EINVAL, "bad arguments");
+  HIP_TRY(hipSetDevice(net->dev));
+  XCS_TRY(mlp_reserve(net, B));
+  HIP_TRY(hipMemcpyAsync(net->X, X, 4 * (size_t)B * MLP_F, hipMemcpyHostToDevice, net->stream));
+  HIP_TRY(hipMemcpyAsync(net->lab, labels, 4 * (size_t)B, hipMemcpyHostToDevice, net->stream));
+  XCS_TRY(mlp_pass(net, B, true, pos_w));
+  if (grad_out) HIP_TRY(hipMemcpyAsync(grad_out, net->g, 4 * (size_t)MLP_NP, hipMemcpyDeviceToHost, net->stream));
+  std::vector<float> l((size_t)B);
+  HIP_TRY(hipMemcpyAsync(l.data(), net->loss, 4 * (size_t)B, hipMemcpyDeviceToHost, net->stream));
+  HIP_TRY(hipStreamSynchronize(net->stream));
+  if (loss_out) {
+    double s = 0.0;
+    for (float v : l) s += v;
+    *loss_out = (float)(s / B);
+  }
+  return XCS_OK;
+}
+
+int xcs_mlp_scores(xcs_mlp *net, const float *X, int N, float *logits) {
+  if (!net || !X || !logits || N <= 0) return set_err(XCS_EINVAL, "bad arguments");
+  HIP_TRY(hipSetDevice(net->dev));
+  XCS_TRY(mlp_reserve(net, N));
+  HIP_TRY(hipMemcpyAsync(net->X, X, 4 * (size_t)N * MLP_F, hipMemcpyHostToDevice, net->stream));
+  XCS_TRY(mlp_pass(net, N, false, 1.0f));
+  HIP_TRY(hipMemcpyAsync(logits, net->logit, 4 * (size_t)N, hipMemcpyDeviceToHost, net->stream));
+  HIP_TRY(hipStreamSynchronize(net->stream));
+  return XCS_OK;
+}
+
+// Training set resident on the device (rows in the order given).
+int xcs_mlp_set_data(xcs_mlp *net, const float *X, const float *labels, long N) {
+  if (!net || !X || !labels || N <= 0) return set_err(XCS_EINVAL, "bad arguments");
+  HIP_TRY(hipSetDevice(net->dev));
+  HIP_TRY(hipStreamSynchronize(net->stream));
+  if (net->dX) (void)hipFree(net->dX);
+  if (net->dL) (void)hipFree(net->dL);
+  net->dX = net->dL = nullptr;
+  HIP_TRY(hipMalloc(&net->dX, 4 * (size_t)N * MLP_F));
+  HIP_TRY(hipMalloc(&net->dL, 4 * (size_t)N));
+  net->allocs.push_back(net->dX);
+  net->allocs.push_back(net->dL);
+  HIP_TRY(hipMemcpy(net->dX, X, 4 * (size_t)N * MLP_F, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(net->dL, labels, 4 * (size_t)N, hipMemcpyHostToDevice));
+  net->ndata = N;
+  return XCS_OK;
+}
+
+// `steps` Adam steps over consecutive batches of the resident training set
+// (wrapping around); *mean_loss = mean loss of the last step's batch.
+int xcs_mlp_train(xcs_mlp *net, int steps, int batch, float lr, float pos_w, float *mean_loss,
+                  float *ms) {
+  if (!net || steps <= 0 || batch <= 0 || !net->dX) return set_err(XCS_EINVAL, "bad arguments or no data");
+  if (batch > net->ndata) return set_err(XCS_EINVAL, "batch larger than the training set");
+  HIP_TRY(hipSetDevice(net->dev));
+  XCS_TRY(mlp_reserve(net, batch));
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  HIP_TRY(hipEventRecord(e0, net->stream));
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  for (int s = 0; s < steps; s++) {
+    const long start = (long)(((net->step) * (long)batch) % (net->ndata - batch + 1));
+    HIP_TRY(hipMemcpyAsync(net->X, net->dX + start * MLP_F, 4 * (size_t)batch * MLP_F,
+                           hipMemcpyDeviceToDevice, net->stream));
+    HIP_TRY(hipMemcpyAsync(net->lab, net->dL + start, 4 * (size_t)batch, hipMemcpyDeviceToDevice,
+                           net->stream));
+    XCS_TRY(mlp_pass(net, batch, true, pos_w));
+    net->step++;
+    const float c1 = 1.0f - powf(b1, (float)net->step), c2 = 1.0f - powf(b2, (float)net->step);
+    hipLaunchKernelGGL(mlp_adam_kernel, dim3(1024), dim3(256), 0, net->stream, net->p, net->m1,
+                       net->m2, net->g, MLP_NP, lr, b1, b2, eps, c1, c2, net->w2b, net->w2tb);
+    HIP_TRY(hipGetLastError());
+  }
+  HIP_TRY(hipEventRecord(e1, net->stream));
+  std::vector<float> l((size_t)batch);
+  HIP_TRY(hipMemcpyAsync(l.data(), net->loss, 4 * (size_t)batch, hipMemcpyDeviceToHost, net->stream));
+  HIP_TRY(hipStreamSynchronize(net->stream));
+  float t = 0.f;
+  HIP_TRY(hipEventElapsedTime(&t, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (ms) *ms = t;
+  if (mean_loss) {
+    double s = 0.0;
+    for (float v : l) s += v;
+    *mean_loss = (float)(s / batch);
+  }
+  return XCS_OK;
+}
+
+// Training samples from a solved many-RHS handle: features of every signal
+// from g = A^T y (FP32), `per` samples per signal (half among the 2k largest
+// |g|, half uniform), label = coordinate in the recovered support.  X_out:
+// count * per * 8, labels_out: count * per (signal s at rows s*per ..).
+int xcs_mrhs_samples(xcs_mrhs *h, int per, unsigned int seed, float *X_out, float *labels_out) {
+  if (!h || per <= 0 || !X_out || !labels_out) return set_err(XCS_EINVAL, "bad arguments");
+  HIP_TRY(hipSetDevice(h->dev));
+  const long S = h->count, n = h->n, m = h->m;
+  float *G, *Xf, *Xo, *Lo;
+  int *pool;
+  std::vector<void *> tmp;
+  auto grab = [&](void **q, size_t bytes) -> int {
+    hipError_t e = hipMalloc(q, bytes);
+    if (e != hipSuccess) return set_err(XCS_ENOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    tmp.push_back(*q);
+    return (int)XCS_OK;
+  };
+  int r = grab((void **)&G, 4 * (size_t)S * n);
+  if (!r) r = grab((void **)&Xf, 4 * (size_t)S * n * MLP_F);
+  if (!r) r = grab((void **)&pool, 4 * (size_t)S * n);
+  if (!r) r = grab((void **)&Xo, 4 * (size_t)S * per * MLP_F);
+  if (!r) r = grab((void **)&Lo, 4 * (size_t)S * per);
+  if (!r) {
+    // G[s*n + j] = sum_i A[j*m + i] y_s[i]
+    hipLaunchKernelGGL(f32_gemm_kernel<false>, dim3((unsigned)((n + 63) / 64), (unsigned)((S + 63) / 64), 1),
+                       dim3(256), 0, h->stream, (int)n, (int)S, (int)m, (const float *)h->A32, m, 1L,
+                       (const void *)h->D.y, m, 1L, G, 1L, n, 1, 0L);
+    hipLaunchKernelGGL(mlp_features_kernel, dim3((unsigned)S), dim3(1024), 0, h->stream, G, (int)n,
+                       (int)m, h->k, Xf);
+    hipLaunchKernelGGL(mlp_samples_kernel, dim3((unsigned)S), dim3(1024), 0, h->stream, G, Xf,
+                       h->D.x, (int)n, h->k, per, (uint32_t)seed, pool, Xo, Lo);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(X_out, Xo, 4 * (size_t)S * per * MLP_F, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(labels_out, Lo, 4 * (size_t)S * per, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) r = set_err(XCS_EHIP, "sample extraction: %s", hipGetErrorString(e));
+  }
+  (void)hipStreamSynchronize(h->stream);
+  for (void *q : tmp) (void)hipFree(q);
+  return r;
+}
+
+// CGIHT on a plan with the learned warm start: g = A^T y, features, network
+// scores, the k best-scored coordinates as the starting support (x = A^T y
+// on it, improved by cg_steps steps of least-squares CG, then CGIHT).  If that solve does not reach tol, plain CGIHT
+// from x = 0 runs as the fallback (*fallback = 1).  *ms = device time of all
+// of it; *warm_ms = the part before the warm CGIHT (product, features,
+// network, top-k).
+int xcs_plan_solve_warm(xcs_plan *pl, xcs_mlp *net, int cg_steps, float *ms, float *warm_ms,
+                        int *fallback) {
+  if (!pl || !ms) return set_err(XCS_EINVAL, "NULL argument");
+  if (cg_steps < 0 || cg_steps > 15) return set_err(XCS_EINVAL, "cg_steps must be in [0, 15]");
+  if (pl->h.alg != ALG_CGIHT) return set_err(XCS_EINVAL, "the warm start is for CGIHT");
+  HIP_TRY(hipSetDevice(pl->dev));
+  const int n = pl->h.n;
+  if (net) {
+    XCS_TRY(mlp_reserve(net, n));
+    HIP_TRY(hipStreamSynchronize(net->stream));
+  }
+  if (!pl->warm_g) {
+    XCS_TRY(pl->alloc(&pl->warm_g, (size_t)n));
+    XCS_TRY(pl->alloc(&pl->warm_mask, (size_t)n));
+    XCS_TRY(pl->alloc(&pl->warm_score, (size_t)n));
+  }
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  HIP_TRY(hipEventRecord(e0, pl->stream));
+  XCS_TRY(plan_upload_desc(pl));
+  XCS_TRY(launch_apply(pl, 1, (const float *)pl->h.y, pl->warm_g, 0));
+  const float *score = pl->warm_score;
+  if (net) {
+    hipLaunchKernelGGL(mlp_features1_kernel, dim3(1), dim3(1024), 0, pl->stream, pl->warm_g, n,
+                       pl->h.m, pl->h.k, net->X);
+    HIP_TRY(hipGetLastError());
+    // the network runs on the plan's stream
+    hipStream_t keep = net->stream;
+    net->stream = pl->stream;
+    const int pr = mlp_pass(net, n, false, 1.0f);
+    net->stream = keep;
+    XCS_TRY(pr);
+    score = net->logit;
+  } else {  // magnitude start: the k largest |A^T y|
+    hipLaunchKernelGGL(mlp_abs_kernel, dim3((n + 255) / 256), dim3(256), 0, pl->stream, pl->warm_g,
+                       n, pl->warm_score);
+  }
+  hipLaunchKernelGGL(mlp_topk_mask_kernel, dim3(1), dim3(1024), 0, pl->stream, score, n,
+                     pl->h.k, pl->warm_mask);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(e1, pl->stream));
+  HIP_TRY(hipEventSynchronize(e1));
+  float t_pre = 0.f;
+  HIP_TRY(hipEventElapsedTime(&t_pre, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  xset(pl->h.warm, pl->warm_mask);
+  pl->h.warm_cg = cg_steps;
+  float t1 = 0.f, t2 = 0.f;
+  const int rw = plan_run(pl, &t1);
+  xset(pl->h.warm, (const int *)nullptr);
+  pl->h.warm_cg = 0;
+  XCS_TRY(rw);
+  // fallback when the warm solve ended above tol and did not reach the FP32
+  // floor of the residual either (10^-5 of ||y||): divergence, cycling or a
+  // stall on a wrong support
+  int fb = 0;
+  const XState &W = pl->last;
+  if (W.stop_reason != 0 && W.err > pl->h.tol && W.err > 1e-5f * W.err_start) {
+    fb = 1;
+    XCS_TRY(plan_run(pl, &t2));
+  }
+  *ms = t_pre + t1 + t2;
+  if (warm_ms) *warm_ms = t_pre;
+  if (fallback) *fallback = fb;
+  return XCS_OK;
+}
+
+// Features of one instance from a host g (kernel-level check): X (n x 8).
+int xcs_mlp_features(const float *g, int n, int m, int k, float *X) {
+  if (!g || !X || n <= 0 || m <= 0 || k <= 0) return set_err(XCS_EINVAL, "bad arguments");
+  XCS_TRY(use_device(0));
+  xcs_mlp tmp;
+  HIP_TRY(hipStreamCreateWithFlags(&tmp.stream, hipStreamNonBlocking));
+  float *dg, *dX;
+  XCS_TRY(tmp.alloc_in(tmp.allocs, &dg, (size_t)n));
+  XCS_TRY(tmp.alloc_in(tmp.allocs, &dX, (size_t)n * MLP_F));
+  HIP_TRY(hipMemcpyAsync(dg, g, 4 * (size_t)n, hipMemcpyHostToDevice, tmp.stream));
+  hipLaunchKernelGGL(mlp_features1_kernel, dim3(1), dim3(1024), 0, tmp.stream, dg, n, m, k, dX);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(X, dX, 4 * (size_t)n * MLP_F, hipMemcpyDeviceToHost, tmp.stream));
+  HIP_TRY(hipStreamSynchronize(tmp.stream));
+  return XCS_OK;
+}
+
+}  // extern "C"

@@ -506,7 +506,8 @@ __device__ void step_restricted_cg(Team &t, Lds &L, const XProblem &P, XState &S
 
 // CG projection on the current support shared by HTP and CSMPSP
 // (algorithms.cu:2607-2669, 3431-3488, 3548-3604).  Returns errCG.
-__device__ float cg_projection(Team &t, Lds &L, const XProblem &P, XState &S, bool fixed_diff) {
+__device__ float cg_projection(Team &t, Lds &L, const XProblem &P, XState &S, bool fixed_diff,
+                               int max_steps = 15) {
   const int kb = S.k_bin;
   float hist[16], evo[16];
 #pragma unroll
@@ -534,7 +535,7 @@ __device__ float cg_projection(Team &t, Lds &L, const XProblem &P, XState &S, bo
   float errCG = (float)v[1];
   hist[15] = errCG;
   int iterCG = 0;
-  while ((errCG > tolCG) & (iterCG < 15) & (diffCG > tol2)) {
+  while ((errCG > tolCG) & (iterCG < max_steps) & (diffCG > tol2)) {
     step_restricted_cg(t, L, P, S, hist);
     errCG = hist[15];
     iterCG++;
@@ -800,6 +801,27 @@ __device__ void solve_instance(Team &t, Lds &L, const XProblem &P, int budget) {
       S.supp_count = 0;
       if (t.rank == 0 && threadIdx.x == 0) *P.st = S;
       return;
+    }
+    if (P.warm && P.alg == ALG_CGIHT) {
+      // Warm start (no counterpart in the reference): x = least squares on the
+      // given support by the restricted CG projection, then the usual start
+      // from that x instead of A^T y.
+      for (int j = t.gtid(); j < n; j += t.gsize()) {
+        const int in = P.warm[j];
+        if (!in) P.x[j] = 0.f;
+        P.bins[j] = in ? 0 : P.num_bins;
+      }
+      t.sync();
+      S.k_bin = 0;
+      S.supp_count = build_support(t, P, 0);
+      (void)cg_projection(t, L, P, S, false, P.warm_cg);
+      S.mu = 0.f;
+      float xw = 0.f;
+      for (int j = t.gtid(); j < n; j += t.gsize()) xw = fmaxf(xw, fabsf(P.x[j]));
+      double w[1] = {xw};
+      const int o1[1] = {1};
+      reduce(t, w, o1);
+      if (w[0] > 0.0) S.max_value = (float)w[0];
     }
     S.slope = (float)(P.num_bins - 1) / S.max_value;
     // FindSupportSet with maxChange 1 (NIHT/CGIHT) or max_value (HTP/CSMPSP):

@@ -185,6 +185,38 @@ int xcs_mrhs_configure(xcs_mrhs *handle, float c_bf16, float c_fp8, int correcti
 int xcs_mrhs_phase_ms(xcs_mrhs *handle, float *ms6);
 void xcs_mrhs_destroy(xcs_mrhs *handle);
 
+/* ---- learned warm start for CGIHT (no counterpart in the reference) ---- */
+/* A per-coordinate MLP (8 features of g = A^T y -> 512 -> 512 -> 1, ReLU; FP32 weights with
+ * BF16 copies for the MFMA layer) scores support membership; trained on the GPU by the
+ * library's own forward/backward kernels with Adam. */
+typedef struct xcs_mlp xcs_mlp;
+int xcs_mlp_create(unsigned int seed, xcs_mlp **net);
+void xcs_mlp_destroy(xcs_mlp *net);
+/* Number of parameters; layout W1[512][8], b1[512], W2[512][512] (out x in), b2[512],
+ * W3[512], b3. */
+int xcs_mlp_param_count(void);
+int xcs_mlp_get_params(xcs_mlp *net, float *params);
+int xcs_mlp_set_params(xcs_mlp *net, const float *params);
+/* Forward + backward on a host batch (X: B x 8, labels 0/1): gradients of the mean weighted
+ * binary cross-entropy (positives weighted pos_w) and the mean loss. */
+int xcs_mlp_grad(xcs_mlp *net, const float *X, const float *labels, int B, float pos_w,
+                 float *grad, float *loss);
+int xcs_mlp_scores(xcs_mlp *net, const float *X, int N, float *logits);
+/* Resident training set and `steps` Adam steps over consecutive batches of it. */
+int xcs_mlp_set_data(xcs_mlp *net, const float *X, const float *labels, long N);
+int xcs_mlp_train(xcs_mlp *net, int steps, int batch, float lr, float pos_w, float *mean_loss,
+                  float *ms);
+/* Training samples from a solved many-RHS handle: `per` coordinates per signal (half among the
+ * 2k largest |A^T y|, half uniform) with their features and label = in the recovered support. */
+int xcs_mrhs_samples(xcs_mrhs *handle, int per, unsigned int seed, float *X, float *labels);
+/* CGIHT on a plan started from the network's k best-scored coordinates: x = A^T y on them,
+ * improved by cg_steps (0..15) steps of least-squares CG on that support, then CGIHT; plain
+ * CGIHT from x = 0 is the fallback when that stops above tol and above 1e-5 ||y|| (the FP32
+ * floor of the residual).  net = NULL starts from the k
+ * largest |A^T y| instead (the same start without the network). */
+int xcs_plan_solve_warm(xcs_plan *plan, xcs_mlp *net, int cg_steps, float *ms, float *warm_ms,
+                        int *fallback);
+
 /* ---- kernel-level entry points (host buffers; used by parity tests) ---- */
 /* One FindSupportSet call (functions.cu:397-436) on the device:
  * in/out state: *min_val, *alpha, *max_bin, *k_bin; out: bins[n], *sum
@@ -203,6 +235,8 @@ int xcs_apply(const char *ens, int transpose, int m, int n, const float *A, cons
  * row-major, host FP32).  fmt "bf16": q = RNE to bf16; "fp8": q = e4m3(v / s) * s with s the
  * row's max |v| / 448.  A_q, B_q (nullable) receive the dequantized operands.  Needs
  * M % 128 == 0 and K * element bytes % 128 == 0. */
+/* The warm-start features of one instance from g = A^T y (host buffers): X (n x 8). */
+int xcs_mlp_features(const float *g, int n, int m, int k, float *X);
 int xcs_gemm_tn(const char *fmt, int M, int N, int K, const float *A, const float *B, float *C,
                 float *A_q, float *B_q);
 int xcs_generate_gen(int m, int n, unsigned int seed, unsigned int instance, float *A);

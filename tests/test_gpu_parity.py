@@ -150,7 +150,13 @@ def _solve_dev(alg, ens, P, y, k, tol, maxiter=None):
     return ga.cs(alg, "dct", y, P.dct_rows + 1, k, P.m, P.n, opts)
 
 
-@pytest.mark.parametrize("alg", ["NIHT", "HTP", "CSMPSP", "CGIHT"])
+CSMPSP_OFF = ("CSMPSP is switched off on the GPU: this case (gen 256x1024, instances 0-2, one "
+              "workgroup) faulted the device once after passing before, a 4-workgroup CSMPSP "
+              "case faulted earlier, and no cause has been shown; the oracle tests keep CSMPSP")
+
+
+@pytest.mark.parametrize("alg", ["NIHT", "HTP", pytest.param("CSMPSP", marks=pytest.mark.skip(
+    reason=CSMPSP_OFF)), "CGIHT"])
 @pytest.mark.parametrize("ens,m,n,k", [("gen", 256, 1024, 20), ("smv", 256, 2048, 20),
                                        ("dct", 256, 1024, 20)])
 def test_solve_matches_oracle(alg, ens, m, n, k, oracle):
@@ -164,3 +170,10 @@ def test_solve_matches_oracle(alg, ens, m, n, k, oracle):
         np.testing.assert_array_equal(xd != 0, x32 != 0)
         assert _rel(xd, x64) < 1e-5
         assert abs(it - r32.iters) <= max(2, r32.iters // 10)
+
+
+def test_csmpsp_is_switched_off():
+    """The switched-off algorithm fails loudly before any device work."""
+    import gaga_amd as ga
+    with pytest.raises(ga.GagaError, match="switched off"):
+        ga.cs("CSMPSP", "gen", 20, 256, 1024, ga.options(tol=1e-5))
