@@ -20,6 +20,8 @@ Also reported (same JSON line):
   many_rhs      512 signals sharing one 4096 x 16384 A: BF16 and FP8 MFMA
                 lock-step CGIHT with refinement, against the FP32
                 one-instance-per-launch form on the same signals
+  warm_start    the learned CGIHT warm start (committed network) against
+                CGIHT without it on fresh subsampled-dct instances
   cpu_baseline  the FP64 CPU oracle (a C restatement of the reference's
                 algorithm) on a bounded sample of the same instances
 """
@@ -71,6 +73,33 @@ def cpu_baseline(sample):
     return dict(value=ok / t_total, unit="problems/s", cores=threads, kind="port",
                 sample="%d CGIHT gen %dx%d k=%d instances, FP64 oracle, %.1f s" % (sample, M, N, K,
                                                                                  t_total))
+
+
+def single_forms(args):
+    """Single-instance forms: the whole-grid A^T r, one instance per cooperative
+    launch (device loop) and the plain form (one outer iteration per launch,
+    host-side stop test)."""
+    import gaga_amd as ga
+    pl = ga.Plan("CGIHT", "gen", K, M, N, 0, ga.options(tol=TOL, seed=SEED, instance=0,
+                                                       vecDistribution="gaussian"))
+    mv1_ms, mv1_bytes = pl.apply_timed(1, 50)
+    pl.close()
+    forms = {}
+    for mode in ("device", "plain"):
+        ok, tsum = 0, 0.0
+        for inst in range(args.plain_instances):
+            p = ga.Plan("CGIHT", "gen", K, M, N, 0, ga.options(
+                tol=TOL, seed=SEED, instance=inst, vecDistribution="gaussian", mode=mode))
+            p.solve()
+            sync()
+            t1 = time.perf_counter()
+            p.solve()
+            sync()
+            tsum += time.perf_counter() - t1
+            ok += int(p.results()["norms"][2] < SUCCESS)
+            p.close()
+        forms["single_" + mode] = dict(problems_per_s=ok / tsum, instances=args.plain_instances)
+    return mv1_ms, mv1_bytes, forms
 
 
 def run_sweep():
@@ -129,7 +158,9 @@ def run_many_rhs():
         out[prec] = dict(solve_ms=ms, wall_s=wall, recovered=int(ok.sum()),
                          problems_per_s=S / wall, recovered_per_s=int(ok.sum()) / wall,
                          lockstep_iterations=[int(v) for v in lp],
-                         phase_ms=[float(v) for v in R.phase_ms()],
+                         phase_ms=dict(zip(("start", "low_precision", "bf16_continuation",
+                                            "refinement", "corrections"),
+                                           [float(v) for v in R.phase_ms()[:5]])),
                          rel_linf_median=float(np.median(r["norms"][:, 2])),
                          resid_rel_median=float(np.median(r["resid_rel"])), gemm=gemm)
         log("many-rhs %s: %.1f ms, %d / %d recovered" % (prec, ms, ok.sum(), S))
@@ -153,6 +184,26 @@ def run_many_rhs():
     return out
 
 
+def run_warm_start():
+    """Learned warm start on fresh dct instances (n = 2^16, m = n/4, k = 0.2 m)."""
+    from gaga_amd import learned
+    if not os.path.exists(learned.WEIGHTS):
+        return None
+    net = learned.load_net()
+    rows = learned.evaluate(net, "dct", 16384, 65536, 3277, instances=6, cg_steps=3)
+    net.close()
+    out = {"workload": "CGIHT dct m=16384 n=65536 k=3277 gaussian, 6 fresh instances"}
+    for key in ("plain", "magnitude", "warm"):
+        out[key] = dict(ms_mean=float(np.mean([r[key][0] for r in rows])),
+                        iterations_mean=float(np.mean([r[key][1] for r in rows])),
+                        recovered=int(sum(r[key][2] for r in rows)))
+    out["fallbacks"] = int(sum(r["fallback"] for r in rows))
+    out["warm_prep_ms_mean"] = float(np.mean([r["warm_prep_ms"] for r in rows]))
+    log("warm start: plain %.1f it, learned %.1f it" % (out["plain"]["iterations_mean"],
+                                                       out["warm"]["iterations_mean"]))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -166,6 +217,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-mrhs", action="store_true")
+    ap.add_argument("--no-warm", action="store_true")
+    ap.add_argument("--no-forms", action="store_true",
+                    help="skip the single-instance forms (cooperative launches)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -204,30 +258,11 @@ def main():
     mv_ms, mv_bytes = B.apply_timed(10)
     B.close()
 
-    pl = ga.Plan("CGIHT", "gen", K, M, N, 0, ga.options(tol=TOL, seed=SEED, instance=0,
-                                                       vecDistribution="gaussian"))
-    mv1_ms, mv1_bytes = pl.apply_timed(1, 50)
-    # single-instance forms: one instance per launch (device loop) and plain
-    # (one outer iteration per launch, host-side stop test)
-    forms = {}
-    for mode in ("device", "plain"):
-        ok, tsum = 0, 0.0
-        for inst in range(args.plain_instances):
-            p = ga.Plan("CGIHT", "gen", K, M, N, 0, ga.options(
-                tol=TOL, seed=SEED, instance=inst, vecDistribution="gaussian", mode=mode))
-            p.solve()
-            sync()
-            t1 = time.perf_counter()
-            p.solve()
-            sync()
-            tsum += time.perf_counter() - t1
-            ok += int(p.results()["norms"][2] < SUCCESS)
-            p.close()
-        forms["single_" + mode] = dict(problems_per_s=ok / tsum, instances=args.plain_instances)
-    pl.close()
+    mv1_ms, mv1_bytes, forms = (None, None, {}) if args.no_forms else single_forms(args)
 
     sweep = None if args.no_sweep else run_sweep()
     mrhs = None if args.no_mrhs else run_many_rhs()
+    warm = None if args.no_warm else run_warm_start()
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_sample)
 
     out = {
@@ -253,7 +288,8 @@ def main():
             "iterations_mean": float(iters.mean()),
             "rel_linf_err_median": float(np.median(linf)),
             "forms": forms,
-            "plain_to_batched_speedup": value / forms["single_plain"]["problems_per_s"],
+            "plain_to_batched_speedup": (value / forms["single_plain"]["problems_per_s"]
+                                         if forms else None),
         },
         "roofline": {
             "bound": "hbm",
@@ -274,10 +310,12 @@ def main():
             "frac": mv_bytes / (mv_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "bytes": mv_bytes,
             "avg_ms": mv_ms,
-            "single_instance_whole_grid_GBs": mv1_bytes / (mv1_ms * 1e-3) / 1e9,
+            "single_instance_whole_grid_GBs": (mv1_bytes / (mv1_ms * 1e-3) / 1e9
+                                               if mv1_ms else None),
         },
         "sweep": sweep,
         "many_rhs": mrhs,
+        "warm_start": warm,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

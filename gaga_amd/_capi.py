@@ -20,7 +20,7 @@ EXPORTS = (
     "xcs_mrhs_gemm_timed", "xcs_mrhs_phase_ms", "xcs_mrhs_fp32_solve", "xcs_mrhs_configure",
     "xcs_mrhs_samples", "xcs_mlp_create", "xcs_mlp_destroy", "xcs_mlp_param_count",
     "xcs_mlp_get_params", "xcs_mlp_set_params", "xcs_mlp_grad", "xcs_mlp_scores",
-    "xcs_mlp_set_data", "xcs_mlp_train", "xcs_plan_solve_warm", "xcs_mlp_features", "xcs_mrhs_destroy",
+    "xcs_mlp_set_data", "xcs_mlp_train", "xcs_plan_solve_warm", "xcs_mlp_features", "xcs_stream_copy", "xcs_mrhs_destroy",
     "xcs_last_error", "xcs_version",
 )
 
@@ -96,6 +96,7 @@ def load():
         L.xcs_mlp_train.argtypes = [_P, _I, _I, ctypes.c_float, ctypes.c_float, _P, _P]
         L.xcs_plan_solve_warm.argtypes = [_P, _P, _I, _P, _P, _P]
         L.xcs_mlp_features.argtypes = [_P, _I, _I, _I, _P]
+        L.xcs_stream_copy.argtypes = [ctypes.c_long, _I, _P]
         L.xcs_mrhs_configure.argtypes = [_P, ctypes.c_float, ctypes.c_float, _I, _I, _I]
         L.xcs_mrhs_fp32_solve.argtypes = [_P, _I, _I, _I, _P, _P, _P]
         L.xcs_mrhs_destroy.argtypes = [_P]

@@ -1,0 +1,26 @@
+"""Plain streaming copy of a known size, for reading HBM traffic counters.
+
+    python -m gaga_amd.calibrate_copy [MiB] [reps]
+
+Each launch reads and writes MiB mebibytes; under `rocprofv3 --pmc
+FETCH_SIZE` (then WRITE_SIZE) the per-dispatch counters of
+stream_copy_kernel against the known size give the scale factor for the
+counters of the other kernels.
+"""
+import ctypes
+import sys
+
+from gaga_amd import _capi
+
+
+def main():
+    mib = int(sys.argv[1]) if len(sys.argv) > 1 else 1024
+    reps = int(sys.argv[2]) if len(sys.argv) > 2 else 5
+    ms = ctypes.c_float(0)
+    _capi._ck(_capi.load().xcs_stream_copy(mib << 20, reps, ctypes.byref(ms)))
+    gbs = 2.0 * (mib << 20) / (ms.value * 1e-3) / 1e9
+    print("stream copy %d MiB: %.3f ms per launch, %.1f GB/s (read + write)" % (mib, ms.value, gbs))
+
+
+if __name__ == "__main__":
+    main()

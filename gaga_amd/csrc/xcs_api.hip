@@ -1728,6 +1728,47 @@ int xcs_batch_phase_ticks(xcs_batch *B, unsigned long long *out16) {
 
 }  // extern "C"
 
+// ------------------------------------------------------------ calibration --
+// Plain streaming copy of known size (float4 loads and stores): the reference
+// point for reading HBM traffic counters (FETCH_SIZE, WRITE_SIZE).
+__global__ void stream_copy_kernel(const float4 *in, float4 *out, long n4) {
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n4;
+       e += (long)gridDim.x * blockDim.x)
+    out[e] = in[e];
+}
+
+extern "C" int xcs_stream_copy(long bytes, int reps, float *ms_per) {
+  if (bytes <= 0 || bytes % 16 || reps <= 0 || !ms_per) return set_err(XCS_EINVAL, "bad arguments");
+  XCS_TRY(use_device(0));
+  void *a = nullptr, *b = nullptr;
+  HIP_TRY(hipMalloc(&a, (size_t)bytes));
+  hipError_t e = hipMalloc(&b, (size_t)bytes);
+  if (e != hipSuccess) {
+    (void)hipFree(a);
+    return set_err(XCS_ENOMEM, "hipMalloc: %s", hipGetErrorString(e));
+  }
+  (void)hipMemset(a, 0, (size_t)bytes);
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  const long n4 = bytes / 16;
+  hipLaunchKernelGGL(stream_copy_kernel, dim3(4096), dim3(256), 0, 0, (const float4 *)a, (float4 *)b, n4);
+  (void)hipEventRecord(e0, 0);
+  for (int r = 0; r < reps; r++)
+    hipLaunchKernelGGL(stream_copy_kernel, dim3(4096), dim3(256), 0, 0, (const float4 *)a, (float4 *)b, n4);
+  (void)hipEventRecord(e1, 0);
+  e = hipEventSynchronize(e1);
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  if (e != hipSuccess) return set_err(XCS_EHIP, "copy: %s", hipGetErrorString(e));
+  *ms_per = ms / reps;
+  return XCS_OK;
+}
+
 // --------------------------------------------------- many right-hand sides --
 #include "xcs_mfma.h"
 #include "xcs_mrhs.h"

@@ -237,6 +237,9 @@ int xcs_apply(const char *ens, int transpose, int m, int n, const float *A, cons
  * M % 128 == 0 and K * element bytes % 128 == 0. */
 /* The warm-start features of one instance from g = A^T y (host buffers): X (n x 8). */
 int xcs_mlp_features(const float *g, int n, int m, int k, float *X);
+/* Calibration: `reps` launches of a plain streaming copy of `bytes` (reads and writes `bytes`
+ * each); *ms_per = average launch time.  Reference point for HBM traffic counters. */
+int xcs_stream_copy(long bytes, int reps, float *ms_per);
 int xcs_gemm_tn(const char *fmt, int M, int N, int K, const float *A, const float *B, float *C,
                 float *A_q, float *B_q);
 int xcs_generate_gen(int m, int n, unsigned int seed, unsigned int instance, float *A);
