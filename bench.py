@@ -56,6 +56,22 @@ def sync():
         pass
 
 
+def measured_traffic(args):
+    """HBM bytes per launch of the solve kernel from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE runs (copy-kernel calibrated), for the profiled
+    configuration only; None otherwise."""
+    if (args.batch, args.block, args.team) != (256, 1024, 1):
+        return None
+    path = os.path.join(ROOT, "profiles", "traffic_solve_batch_kernel.txt")
+    try:
+        for line in open(path):
+            if line.startswith("bytes_per_launch"):
+                return float(line.split()[1])
+    except OSError:
+        return None
+    return None
+
+
 def cpu_baseline(sample):
     """FP64 oracle on `sample` instances of the workload (bounded CPU work)."""
     from oracle import oracle as O
@@ -298,7 +314,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(args),
             "algorithmic_bytes_per_launch": bytes_per_launch,
             "avg_launch_ms": mean_ms,
         },
