@@ -3,8 +3,9 @@
 Gaussian nonzeros, FP32), one MI355X.
 
 A step is one pass of the hot path over one batch of synthetic input: one
-persistent launch that solves `--batch` resident instances (one workgroup per
-instance) from x = 0 to the stopping rule.  value = recovered instances
+persistent launch that solves `--batch` resident instances (default 640, 40 GB
+of matrices; 256 workgroups of 1024 threads take them from a device queue)
+from x = 0 to the stopping rule.  value = recovered instances
 (relative l_inf error < 1e-3, GAGA/CGIHTcode/shared/tests_delta_fixed.m:17-23)
 per second of timed wall clock, inputs already resident in HBM.
 
@@ -60,7 +61,7 @@ def measured_traffic(args):
     """HBM bytes per launch of the solve kernel from the committed rocprofv3
     FETCH_SIZE/WRITE_SIZE runs (copy-kernel calibrated), for the profiled
     configuration only; None otherwise."""
-    if (args.batch, args.block, args.team) != (256, 1024, 1):
+    if (args.batch, args.block, args.team) != (640, 1024, 1):
         return None
     path = os.path.join(ROOT, "profiles", "traffic_solve_batch_kernel.txt")
     try:
@@ -129,8 +130,8 @@ def run_sweep():
     B = ga.Batch("CGIHT", specs, ga.options(tol=1e-4, maxiter=500, seed=SEED,
                                             vecDistribution="binary"))
     create_s = time.perf_counter() - t0
-    team, block, probe_ms = B.tune(probe_iters=32, cache=os.path.join(ROOT, "gaga_amd",
-                                                                     "tune_cache.txt"))
+    team, block, big, probe_ms = B.tune(probe_iters=0, cache=os.path.join(ROOT, "gaga_amd",
+                                                                          "tune_cache.txt"))
     B.solve()  # warm
     sync()
     t0 = time.perf_counter()
@@ -145,7 +146,8 @@ def run_sweep():
                 instances=len(specs), solve_ms=ms, wall_s=wall, recovered=int(ok.sum()),
                 problems_per_s=len(specs) / wall, recovered_per_s=int(ok.sum()) / wall,
                 algorithmic_GBs=byt / (ms * 1e-3) / 1e9, hbm_frac=byt / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                tuned={"team_size": team, "block": block, "probe_ms": probe_ms},
+                tuned={"team_size": team, "block": block, "team_instances": big,
+                       "probe_ms": probe_ms},
                 generation_s=create_s)
 
 
@@ -179,7 +181,19 @@ def run_many_rhs():
                                            [float(v) for v in R.phase_ms()[:5]])),
                          rel_linf_median=float(np.median(r["norms"][:, 2])),
                          resid_rel_median=float(np.median(r["resid_rel"])), gemm=gemm)
-        log("many-rhs %s: %.1f ms, %d / %d recovered" % (prec, ms, ok.sum(), S))
+        # error against the FP64 oracle (a CPU restatement of the reference's
+        # CGIHT) on the same A and y, first 4 signals
+        from oracle import oracle as O
+        if "oracle_problem" not in out:
+            out["oracle_problem"] = O.make_problem("gen", m, n, SEED, 9)
+        diffs = []
+        for sgl in range(4):
+            x, xt, yv = R.x(sgl)
+            xo, _, _, _ = O.solve("CGIHT", out["oracle_problem"], yv, k, tol=TOL, precision=64)
+            diffs.append(float(np.linalg.norm(x - xo) / np.linalg.norm(xo)))
+        out[prec]["rel_l2_vs_fp64_oracle"] = diffs
+        log("many-rhs %s: %.1f ms, %d / %d recovered, vs FP64 oracle %s" % (prec, ms, ok.sum(), S,
+                                                                          diffs))
         if prec == "bf16":
             sample = 16
             res = {}
@@ -195,6 +209,7 @@ def run_many_rhs():
             base = res
             out["fp32_one_instance_per_launch"] = res
         R.close()
+    out.pop("oracle_problem", None)
     best = max(out["bf16"]["recovered_per_s"], out["fp8"]["recovered_per_s"])
     out["speedup_vs_fp32_plain"] = best / max(base["plain"]["recovered_per_s"], 1e-9)
     return out
@@ -225,7 +240,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=256, help="instances per step")
+    ap.add_argument("--batch", type=int, default=640, help="instances per step")
     ap.add_argument("--block", type=int, default=1024)
     ap.add_argument("--team", type=int, default=1)
     ap.add_argument("--plain-instances", type=int, default=8)
@@ -236,6 +251,8 @@ def main():
     ap.add_argument("--no-warm", action="store_true")
     ap.add_argument("--no-forms", action="store_true",
                     help="skip the single-instance forms (cooperative launches)")
+    ap.add_argument("--only", choices=("sweep", "mrhs", "warm"), default=None,
+                    help="run one secondary workload alone and print its JSON (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -244,6 +261,10 @@ def main():
 
     import gaga_amd as ga
     ga.load()
+    if args.only:
+        fn = {"sweep": run_sweep, "mrhs": run_many_rhs, "warm": run_warm_start}[args.only]
+        print(json.dumps({args.only: fn()}))
+        return
     opts = ga.options(tol=TOL, seed=SEED, vecDistribution="gaussian", teamSize=args.team,
                       blockSize=args.block)
     specs = [dict(ens="gen", k=K, m=M, n=N, instance=i) for i in range(args.batch)]

@@ -18,7 +18,7 @@ EXPORTS = (
     "xcs_find_support_set", "xcs_apply", "xcs_generate_gen", "xcs_gemm_tn",
     "xcs_mrhs_create", "xcs_mrhs_solve", "xcs_mrhs_results", "xcs_mrhs_x",
     "xcs_mrhs_gemm_timed", "xcs_mrhs_phase_ms", "xcs_mrhs_fp32_solve", "xcs_mrhs_configure",
-    "xcs_mrhs_samples", "xcs_mlp_create", "xcs_mlp_destroy", "xcs_mlp_param_count",
+    "xcs_mrhs_samples", "xcs_mrhs_gemm_tile", "xcs_mlp_create", "xcs_mlp_destroy", "xcs_mlp_param_count",
     "xcs_mlp_get_params", "xcs_mlp_set_params", "xcs_mlp_grad", "xcs_mlp_scores",
     "xcs_mlp_set_data", "xcs_mlp_train", "xcs_plan_solve_warm", "xcs_mlp_features", "xcs_stream_copy", "xcs_mrhs_destroy",
     "xcs_last_error", "xcs_version",
@@ -75,7 +75,7 @@ def load():
         L.xcs_batch_apply_timed.argtypes = [_P, _I, _P, _P]
         L.xcs_batch_create.argtypes = [_C, _I, _P, _P, _P, _P, _P, _P, _P, _P]
         L.xcs_batch_solve.argtypes = [_P, _P]
-        L.xcs_batch_configure.argtypes = [_P, _I, _I]
+        L.xcs_batch_configure.argtypes = [_P, _I, _I, _I]
         L.xcs_gemm_tn.argtypes = [_C, _I, _I, _I, _P, _P, _P, _P, _P]
         L.xcs_mrhs_create.argtypes = [_I, _I, _I, _I, _C, _P, _P]
         L.xcs_mrhs_solve.argtypes = [_P, _P, _P]
@@ -84,6 +84,7 @@ def load():
         L.xcs_mrhs_gemm_timed.argtypes = [_P, _I, _C, _I, _P, _P]
         L.xcs_mrhs_phase_ms.argtypes = [_P, _P]
         L.xcs_mrhs_samples.argtypes = [_P, _I, ctypes.c_uint, _P, _P]
+        L.xcs_mrhs_gemm_tile.argtypes = [_P, _I]
         L.xcs_mlp_create.argtypes = [ctypes.c_uint, _P]
         L.xcs_mlp_destroy.argtypes = [_P]
         L.xcs_mlp_destroy.restype = None
@@ -101,7 +102,7 @@ def load():
         L.xcs_mrhs_fp32_solve.argtypes = [_P, _I, _I, _I, _P, _P, _P]
         L.xcs_mrhs_destroy.argtypes = [_P]
         L.xcs_mrhs_destroy.restype = None
-        L.xcs_batch_tune.argtypes = [_P, _I, _C, _P, _P, _P]
+        L.xcs_batch_tune.argtypes = [_P, _I, _C, _I, _P, _P, _P, _P]
         L.xcs_batch_results.argtypes = [_P, _P, _P, _P, _P]
         L.xcs_batch_x.argtypes = [_P, _I, _P, _P]
         L.xcs_batch_info.argtypes = [_P, _P, _P, _P, _P]
@@ -347,17 +348,20 @@ class Batch:
         _ck(load().xcs_batch_apply_timed(self._h, int(reps), ctypes.byref(ms), ctypes.byref(b)))
         return ms.value, b.value
 
-    def configure(self, team_size, block_size):
-        _ck(load().xcs_batch_configure(self._h, int(team_size), int(block_size)))
+    def configure(self, team_size, block_size, big_count=-1):
+        """team_size workgroups per instance for the big_count largest instances
+        (-1: all), one workgroup for the rest."""
+        _ck(load().xcs_batch_configure(self._h, int(team_size), int(block_size), int(big_count)))
 
-    def tune(self, probe_iters=32, cache=None):
+    def tune(self, probe_iters=32, cache=None, verbose=False):
         """Pick the team geometry by timing candidates on this batch; returns
-        (team_size, block_size, probe_ms)."""
-        c, b, t = ctypes.c_int(), ctypes.c_int(), ctypes.c_float()
+        (team_size, block_size, big_count, probe_ms)."""
+        c, b, g, t = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_float()
         path = cache.encode() if cache else None
-        _ck(load().xcs_batch_tune(self._h, int(probe_iters), path, ctypes.byref(c),
-                                  ctypes.byref(b), ctypes.byref(t)))
-        return c.value, b.value, t.value
+        _ck(load().xcs_batch_tune(self._h, int(probe_iters), path, 1 if verbose else 0,
+                                  ctypes.byref(c), ctypes.byref(b), ctypes.byref(g),
+                                  ctypes.byref(t)))
+        return c.value, b.value, g.value, t.value
 
     def info(self):
         t, c, b, g = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_float()
@@ -449,6 +453,9 @@ class ManyRHS:
                   ref_inner=4):
         _ck(load().xcs_mrhs_configure(self._h, float(c_bf16), float(c_fp8), int(corrections),
                                       int(ref_outer), int(ref_inner)))
+
+    def gemm_tile(self, big):
+        _ck(load().xcs_mrhs_gemm_tile(self._h, 1 if big else 0))
 
     def samples(self, per, seed=1):
         """Training samples of every signal: (X (count*per x 8), labels)."""

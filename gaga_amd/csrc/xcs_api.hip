@@ -71,29 +71,63 @@ __device__ __forceinline__ void batch_team(int c, int *team_id, int *rank) {
   }
 }
 
-// Batched persistent form: teams of c workgroups take instances from a
-// queue and solve each to completion with team-scoped barriers only.
+// The workgroup of rank 0 in this workgroup's team (see batch_team).
+__device__ __forceinline__ int batch_team_leader(int c) {
+  const int b = blockIdx.x, nb = gridDim.x;
+  if (c == 1) return b;
+  if (nb % (8 * c) == 0) {
+    const int xcd = b % 8, local = b / 8;
+    return (local / c) * c * 8 + xcd;
+  }
+  return (b / c) * c;
+}
+
+// Batched persistent form, two queues over the instances (largest first):
+// teams of c workgroups take instances [0, n_big) from queue[0] and solve each
+// with team-scoped barriers; when that range is used up each team dissolves
+// and its workgroups take [n_big, count) one per workgroup from queue[1].  A
+// team works in its leader's workspace and a lone workgroup in its own, so a
+// dissolved workgroup never shares scratch with a team still running.
 __global__ void __launch_bounds__(1024) solve_batch_kernel(const XProblem *Ps, int count,
-                                                           int *queue, const XWork *works, int *slots,
+                                                           int n_big, int *queue,
+                                                           const XWork *works, int *slots,
                                                            double *red, unsigned *bar, int c,
                                                            int budget) {
   __shared__ Lds L;
   __shared__ XProblem sP;
-  int team_id, rank;
-  batch_team(c, &team_id, &rank);
-  Team t{rank, c, red + (size_t)team_id * 2 * RED_SLOTS * c, 0, bar + (size_t)team_id * BAR_WORDS};
+  __shared__ int sidx;
+  if (c > 1) {
+    int team_id, rank;
+    batch_team(c, &team_id, &rank);
+    const int leader = batch_team_leader(c);
+    Team t{rank, c, red + (size_t)team_id * 2 * RED_SLOTS * c, 0, bar + (size_t)team_id * BAR_WORDS};
+    while (true) {
+      if (rank == 0 && threadIdx.x == 0) slots[team_id] = atomicAdd(&queue[0], 1);
+      t.sync();
+      const int idx = slots[team_id];
+      if (idx >= n_big) break;
+      if (threadIdx.x == 0) {
+        sP = Ps[idx];
+        attach_work(sP, works[leader]);
+      }
+      __syncthreads();
+      solve_instance(t, L, sP, budget);
+      t.sync();
+    }
+  }
+  Team t1{0, 1, nullptr, 0, nullptr};
   while (true) {
-    if (rank == 0 && threadIdx.x == 0) slots[team_id] = atomicAdd(queue, 1);
-    t.sync();
-    const int idx = slots[team_id];
+    if (threadIdx.x == 0) sidx = n_big + atomicAdd(&queue[1], 1);
+    __syncthreads();
+    const int idx = sidx;
     if (idx >= count) break;
     if (threadIdx.x == 0) {
       sP = Ps[idx];
-      attach_work(sP, works[team_id]);
+      attach_work(sP, works[blockIdx.x]);
     }
     __syncthreads();
-    solve_instance(t, L, sP, budget);
-    t.sync();
+    solve_instance(t1, L, sP, budget);
+    __syncthreads();
   }
 }
 
@@ -168,7 +202,10 @@ __global__ void batch_reset_kernel(float *xs, size_t xs_len, int *st, size_t st_
   for (size_t i = i0; i < xs_len; i += stride) xs[i] = 0.f;
   for (size_t i = i0; i < st_words; i += stride) st[i] = 0;
   for (size_t i = i0; i < bar_words; i += stride) bar[i] = 0u;
-  if (i0 == 0) *queue = 0;
+  if (i0 == 0) {
+    queue[0] = 0;
+    queue[1] = 0;
+  }
 }
 
 // One FindSupportSet call with the team binning (functions.cu:397-436).
@@ -700,6 +737,7 @@ struct xcs_batch {
   int dev = 0;
   hipStream_t stream = nullptr;
   int alg = 0, count = 0, c = 1, block = 256, nteams = 0, grid = 0, budget = 0;
+  int n_big = 0;  // instances [0, n_big) (queue order) go to teams of c workgroups
   int n_max = 1, lda_max = 64, bins_max = 1000;
   bool any_dct = false;
   std::vector<XProblem> hP;  // queue order
@@ -788,7 +826,9 @@ static int batch_occupancy(int block, int *per_cu, int *cus) {
 }
 
 // Team geometry and per-team workspace for c workgroups of `block` threads.
-static int batch_configure(xcs_batch *B, int c, int block) {
+// big < 0: every instance goes to a team of c workgroups; otherwise the
+// `big` largest do and the rest run one per workgroup.
+static int batch_configure(xcs_batch *B, int c, int block, int big = -1) {
   if (block % 64 || block < 64 || block > 1024)
     return set_err(XCS_EINVAL, "block_size must be a multiple of 64 in [64, 1024]");
   if (c < 1) return set_err(XCS_EINVAL, "team_size must be >= 1");
@@ -806,7 +846,9 @@ static int batch_configure(xcs_batch *B, int c, int block) {
   B->block = block;
   B->nteams = nteams;
   B->grid = nteams * c;
-  const size_t T = (size_t)nteams, nn = (size_t)B->n_max, ml = (size_t)B->lda_max;
+  B->n_big = c == 1 ? 0 : (big < 0 ? B->count : std::min(big, B->count));
+  // one workspace per workgroup (a team uses its leader's)
+  const size_t T = (size_t)B->grid, nn = (size_t)B->n_max, ml = (size_t)B->lda_max;
   float *f_n, *f_m, *part;
   int *i_n, *cnt, *wgc;
   float2 *cw = nullptr;
@@ -839,8 +881,8 @@ static int batch_configure(xcs_batch *B, int c, int block) {
   HIP_TRY(hipMemcpyAsync(B->dW, hw.data(), sizeof(XWork) * T, hipMemcpyHostToDevice, B->stream));
   XCS_TRY(B->walloc(&B->slots, T));
   XCS_TRY(B->walloc(&B->queue, 64));
-  XCS_TRY(B->walloc(&B->red, T * 2 * RED_SLOTS * (size_t)c));
-  XCS_TRY(B->walloc(&B->bar, T * BAR_WORDS));
+  XCS_TRY(B->walloc(&B->red, (size_t)nteams * 2 * RED_SLOTS * (size_t)c));
+  XCS_TRY(B->walloc(&B->bar, (size_t)nteams * BAR_WORDS));
   HIP_TRY(hipStreamSynchronize(B->stream));  // hw goes out of scope
   return XCS_OK;
 }
@@ -1076,9 +1118,9 @@ static int batch_enqueue(xcs_batch *B) {
                      (int *)B->sts, sizeof(XState) / sizeof(int) * (size_t)B->count, B->queue,
                      B->bar, (size_t)BAR_WORDS * B->nteams);
   HIP_TRY(hipGetLastError());
-  void *args[] = {(void *)&B->dP, (void *)&B->count, (void *)&B->queue, (void *)&B->dW,
-                  (void *)&B->slots, (void *)&B->red, (void *)&B->bar, (void *)&B->c,
-                  (void *)&B->budget};
+  void *args[] = {(void *)&B->dP, (void *)&B->count, (void *)&B->n_big, (void *)&B->queue,
+                  (void *)&B->dW, (void *)&B->slots, (void *)&B->red, (void *)&B->bar,
+                  (void *)&B->c, (void *)&B->budget};
   if (B->c > 1) {
     HIP_TRY(hipLaunchCooperativeKernel((const void *)solve_batch_kernel, dim3(B->grid),
                                        dim3(B->block), args, 0, B->stream));
@@ -1167,26 +1209,27 @@ static std::string batch_signature(const xcs_batch *B, int probe_iters) {
   return s + ":probe=" + std::to_string(probe_iters);
 }
 
-static int batch_tune(xcs_batch *B, int probe_iters, const char *cache_path, int *team_out,
-                      int *block_out, float *ms_out) {
+static int batch_tune(xcs_batch *B, int probe_iters, const char *cache_path, int verbose,
+                      int *team_out, int *block_out, int *big_out, float *ms_out) {
   const std::string sig = batch_signature(B, probe_iters);
   if (cache_path && *cache_path) {
     FILE *f = fopen(cache_path, "r");
     if (f) {
       char key[1024];
-      int c, b;
+      int c, b, big;
       float t;
       bool hit = false;
-      while (fscanf(f, "%1023s %d %d %f", key, &c, &b, &t) == 4)
+      while (fscanf(f, "%1023s %d %d %d %f", key, &c, &b, &big, &t) == 5)
         if (sig == key) {
           hit = true;
           *team_out = c;
           *block_out = b;
+          *big_out = big;
           *ms_out = t;
         }
       fclose(f);
       if (hit) {
-        XCS_TRY(batch_configure(B, *team_out, *block_out));
+        XCS_TRY(batch_configure(B, *team_out, *block_out, *big_out));
         return XCS_OK;
       }
     }
@@ -1195,36 +1238,47 @@ static int batch_tune(xcs_batch *B, int probe_iters, const char *cache_path, int
   const int budget = probe_iters > 0 ? probe_iters : saved_budget;
   static const int blocks[] = {256, 512, 1024};
   static const int teams[] = {1, 2, 4, 8};
+  // teams for every instance, or for the largest 1/16, 1/8, 1/4 only
+  const int splits[] = {-1, B->count / 16, B->count / 8, B->count / 4};
   float best = 1e30f;
-  int best_c = B->c, best_b = B->block;
+  int best_c = B->c, best_b = B->block, best_big = -1;
   for (int bi = 0; bi < 3; bi++)
-    for (int ti = 0; ti < 4; ti++) {
-      const int c = teams[ti], blk = blocks[bi];
-      if (c > 1 && B->alg == ALG_CSMPSP) continue;
-      int per_cu, cus;
-      XCS_TRY(batch_occupancy(blk, &per_cu, &cus));
-      if (c > 1 && per_cu * cus / c < 8) continue;
-      XCS_TRY(batch_configure(B, c, blk));
-      B->budget = budget;
-      float ms = 0.f;
-      const int rc = batch_run(B, &ms);
-      B->budget = saved_budget;
-      if (rc != XCS_OK) return rc;
-      if (ms < best) {
-        best = ms;
-        best_c = c;
-        best_b = blk;
+    for (int ti = 0; ti < 4; ti++)
+      for (int si = 0; si < 4; si++) {
+        const int c = teams[ti], blk = blocks[bi], big = splits[si];
+        if (c == 1 && si > 0) continue;
+        if (c > 1 && si > 0 && big < 1) continue;
+        int per_cu, cus;
+        XCS_TRY(batch_occupancy(blk, &per_cu, &cus));
+        if (c > 1 && per_cu * cus / c < 8) continue;
+        XCS_TRY(batch_configure(B, c, blk, big));
+        B->budget = budget;
+        float ms = 0.f;
+        const int rc = batch_run(B, &ms);
+        B->budget = saved_budget;
+        if (rc != XCS_OK) return rc;
+        if (verbose) {
+          fprintf(stderr, "tune: team %d block %4d team instances %5d: %9.2f ms\n", c, blk,
+                  B->n_big, ms);
+          fflush(stderr);
+        }
+        if (ms < best) {
+          best = ms;
+          best_c = c;
+          best_b = blk;
+          best_big = big;
+        }
       }
-    }
   B->budget = saved_budget;
-  XCS_TRY(batch_configure(B, best_c, best_b));
+  XCS_TRY(batch_configure(B, best_c, best_b, best_big));
   *team_out = best_c;
   *block_out = best_b;
+  *big_out = best_big;
   *ms_out = best;
   if (cache_path && *cache_path) {
     FILE *f = fopen(cache_path, "a");
     if (f) {
-      fprintf(f, "%s %d %d %.4f\n", sig.c_str(), best_c, best_b, best);
+      fprintf(f, "%s %d %d %d %.4f\n", sig.c_str(), best_c, best_b, best_big, best);
       fclose(f);
     }
   }
@@ -1650,18 +1704,20 @@ int xcs_batch_x(xcs_batch *B, int idx, float *x_out, float *x_true) {
   return XCS_OK;
 }
 
-int xcs_batch_configure(xcs_batch *B, int team_size, int block_size) {
+int xcs_batch_configure(xcs_batch *B, int team_size, int block_size, int big_count) {
   if (!B) return set_err(XCS_EINVAL, "NULL batch");
   HIP_TRY(hipSetDevice(B->dev));
-  return batch_configure(B, team_size, block_size);
+  return batch_configure(B, team_size, block_size, big_count);
 }
 
-int xcs_batch_tune(xcs_batch *B, int probe_iters, const char *cache_path, int *team_size,
-                   int *block_size, float *probe_ms) {
-  if (!B || !team_size || !block_size || !probe_ms) return set_err(XCS_EINVAL, "NULL argument");
+int xcs_batch_tune(xcs_batch *B, int probe_iters, const char *cache_path, int verbose,
+                   int *team_size, int *block_size, int *big_count, float *probe_ms) {
+  if (!B || !team_size || !block_size || !big_count || !probe_ms)
+    return set_err(XCS_EINVAL, "NULL argument");
   if (probe_iters < 0) return set_err(XCS_EINVAL, "probe_iters must be >= 0");
   HIP_TRY(hipSetDevice(B->dev));
-  return batch_tune(B, probe_iters, cache_path, team_size, block_size, probe_ms);
+  return batch_tune(B, probe_iters, cache_path, verbose, team_size, block_size, big_count,
+                    probe_ms);
 }
 
 int xcs_batch_info(xcs_batch *B, int *teams, int *team_size, int *block, float *gen_ms) {

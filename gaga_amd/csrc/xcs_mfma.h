@@ -9,10 +9,11 @@
 // FP8 e4m3 (v_mfma_f32_32x32x64_f8f6f4, format code 0 for both operands);
 // accumulation in FP32.
 //
-// Tiling: 256 threads = 4 waves in a 2 x 2 layout, each wave a 64 x 64 tile
-// made of 2 x 2 MFMA 32 x 32 blocks; 128 x 128 block tile; 128 bytes of K per
-// stage (64 bf16 or 128 fp8), two LDS stages (72 KB) with the next stage
-// prefetched into registers while the current one is multiplied.
+// Tiling: 256 threads = 4 waves in a 2 x 2 layout, each wave WM x WN MFMA
+// 32 x 32 blocks (block tile 64 WM x 64 WN; 2 x 2 -> 128 x 128, 4 x 2 ->
+// 256 x 128); 128 bytes of K per stage (64 bf16 or 128 fp8), two LDS stages
+// with the next stage prefetched into registers while the current one is
+// multiplied.
 //
 // MFMA operand placement used here: lane l holds row (l % 32) of the A block
 // and column (l % 32) of the B block, with the same run of K values (bytes
@@ -56,11 +57,15 @@ struct GemmArgs {
 
 __device__ __forceinline__ int gemm_esize(int fmt) { return fmt == FMT_BF16 ? 2 : 1; }
 
-template <int FMT>
+// WM x WN MFMA 32 x 32 blocks per wave; the four waves sit 2 x 2, so the
+// block tile is (64 WM) x (64 WN).  LDS: two stages of (64 WM + 64 WN) rows.
+template <int FMT, int WM, int WN>
 __global__ void __launch_bounds__(GT_THREADS) gemm_tn_kernel(GemmArgs g) {
-  __shared__ __attribute__((aligned(16))) uint8_t lds[2][2][GT_M * GT_ROW];
+  constexpr int BM = 64 * WM, BN = 64 * WN;
+  constexpr int CA = BM * 8 / GT_THREADS, CB = BN * 8 / GT_THREADS;  // 16-byte chunks per thread
+  __shared__ __attribute__((aligned(16))) uint8_t lds[2][(BM + BN) * GT_ROW];
   const int ncols = g.ncols ? *g.ncols : g.N;
-  const int m0 = blockIdx.x * GT_M, n0 = blockIdx.y * GT_N;
+  const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
   if (n0 >= ncols) return;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const long kbytes = (long)g.K * (FMT == FMT_BF16 ? 2 : 1);
@@ -68,40 +73,44 @@ __global__ void __launch_bounds__(GT_THREADS) gemm_tn_kernel(GemmArgs g) {
   const long kb0 = (long)blockIdx.z * kb_split;
   const int nsteps = (int)(kb_split / GT_KB);
 
-  // global -> register assignment: 4 chunks of 16 bytes per thread per operand
-  const uint8_t *pa[4], *pb[4];
-  bool okb[4];
+  // global -> register assignment: chunks of 16 bytes, 8 per 128-byte row
+  const uint8_t *pa[CA], *pb[CB];
+  bool okb[CB];
 #pragma unroll
-  for (int i = 0; i < 4; i++) {
+  for (int i = 0; i < CA; i++) {
     const int row = (tid >> 3) + 32 * i;
     pa[i] = g.A + (long)(m0 + row) * g.lda_bytes + kb0 + (tid & 7) * 16;
+  }
+#pragma unroll
+  for (int i = 0; i < CB; i++) {
+    const int row = (tid >> 3) + 32 * i;
     const int c = n0 + row;
     okb[i] = c < ncols;
     const int phys = okb[i] ? (g.col_map ? g.col_map[c] : c) : 0;
     pb[i] = g.B + (long)phys * g.ldb_bytes + kb0 + (tid & 7) * 16;
   }
-  i32x4_t ra[4], rb[4];
+  i32x4_t ra[CA], rb[CB];
   auto gload = [&](long off) {
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-      ra[i] = *(const i32x4_t *)(pa[i] + off);
+    for (int i = 0; i < CA; i++) ra[i] = *(const i32x4_t *)(pa[i] + off);
+#pragma unroll
+    for (int i = 0; i < CB; i++)
       rb[i] = okb[i] ? *(const i32x4_t *)(pb[i] + off) : i32x4_t{0, 0, 0, 0};
-    }
   };
   auto lstore = [&](int buf) {
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-      const int off = ((tid >> 3) + 32 * i) * GT_ROW + (tid & 7) * 16;
-      *(i32x4_t *)(&lds[buf][0][off]) = ra[i];
-      *(i32x4_t *)(&lds[buf][1][off]) = rb[i];
-    }
+    for (int i = 0; i < CA; i++)
+      *(i32x4_t *)(&lds[buf][((tid >> 3) + 32 * i) * GT_ROW + (tid & 7) * 16]) = ra[i];
+#pragma unroll
+    for (int i = 0; i < CB; i++)
+      *(i32x4_t *)(&lds[buf][(BM + (tid >> 3) + 32 * i) * GT_ROW + (tid & 7) * 16]) = rb[i];
   };
 
-  f32x16_t acc[2][2];
+  f32x16_t acc[WM][WN];
 #pragma unroll
-  for (int a = 0; a < 2; a++)
+  for (int a = 0; a < WM; a++)
 #pragma unroll
-    for (int b = 0; b < 2; b++)
+    for (int b = 0; b < WN; b++)
 #pragma unroll
       for (int v = 0; v < 16; v++) acc[a][b][v] = 0.f;
 
@@ -112,39 +121,43 @@ __global__ void __launch_bounds__(GT_THREADS) gemm_tn_kernel(GemmArgs g) {
   for (int s = 0; s < nsteps; s++) {
     const int buf = s & 1;
     if (s + 1 < nsteps) gload((long)(s + 1) * GT_KB);
-    const uint8_t *LA = lds[buf][0], *LB = lds[buf][1];
+    const uint8_t *LA = lds[buf], *LB = lds[buf] + BM * GT_ROW;
     if (FMT == FMT_BF16) {
 #pragma unroll
       for (int kk = 0; kk < 4; kk++) {
-        bf16x8_t fa[2], fb[2];
+        bf16x8_t fa[WM], fb[WN];
 #pragma unroll
-        for (int i = 0; i < 2; i++) {
-          fa[i] = *(const bf16x8_t *)(LA + (wm * 64 + i * 32 + r32) * GT_ROW + kk * 32 + h * 16);
-          fb[i] = *(const bf16x8_t *)(LB + (wn * 64 + i * 32 + r32) * GT_ROW + kk * 32 + h * 16);
-        }
+        for (int i = 0; i < WM; i++)
+          fa[i] = *(const bf16x8_t *)(LA + (wm * 32 * WM + i * 32 + r32) * GT_ROW + kk * 32 + h * 16);
 #pragma unroll
-        for (int a = 0; a < 2; a++)
+        for (int i = 0; i < WN; i++)
+          fb[i] = *(const bf16x8_t *)(LB + (wn * 32 * WN + i * 32 + r32) * GT_ROW + kk * 32 + h * 16);
 #pragma unroll
-          for (int b = 0; b < 2; b++)
+        for (int a = 0; a < WM; a++)
+#pragma unroll
+          for (int b = 0; b < WN; b++)
             acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[a], fb[b], acc[a][b], 0, 0, 0);
       }
     } else {
 #pragma unroll
       for (int kk = 0; kk < 2; kk++) {
-        i32x8_t fa[2], fb[2];
+        i32x8_t fa[WM], fb[WN];
 #pragma unroll
-        for (int i = 0; i < 2; i++) {
-          const uint8_t *qa = LA + (wm * 64 + i * 32 + r32) * GT_ROW + kk * 64 + h * 32;
-          const uint8_t *qb = LB + (wn * 64 + i * 32 + r32) * GT_ROW + kk * 64 + h * 32;
-          const i32x4_t a0 = *(const i32x4_t *)qa, a1 = *(const i32x4_t *)(qa + 16);
-          const i32x4_t b0 = *(const i32x4_t *)qb, b1 = *(const i32x4_t *)(qb + 16);
-          fa[i] = i32x8_t{a0[0], a0[1], a0[2], a0[3], a1[0], a1[1], a1[2], a1[3]};
-          fb[i] = i32x8_t{b0[0], b0[1], b0[2], b0[3], b1[0], b1[1], b1[2], b1[3]};
+        for (int i = 0; i < WM; i++) {
+          const uint8_t *q = LA + (wm * 32 * WM + i * 32 + r32) * GT_ROW + kk * 64 + h * 32;
+          const i32x4_t x0 = *(const i32x4_t *)q, x1 = *(const i32x4_t *)(q + 16);
+          fa[i] = i32x8_t{x0[0], x0[1], x0[2], x0[3], x1[0], x1[1], x1[2], x1[3]};
         }
 #pragma unroll
-        for (int a = 0; a < 2; a++)
+        for (int i = 0; i < WN; i++) {
+          const uint8_t *q = LB + (wn * 32 * WN + i * 32 + r32) * GT_ROW + kk * 64 + h * 32;
+          const i32x4_t x0 = *(const i32x4_t *)q, x1 = *(const i32x4_t *)(q + 16);
+          fb[i] = i32x8_t{x0[0], x0[1], x0[2], x0[3], x1[0], x1[1], x1[2], x1[3]};
+        }
 #pragma unroll
-          for (int b = 0; b < 2; b++)
+        for (int a = 0; a < WM; a++)
+#pragma unroll
+          for (int b = 0; b < WN; b++)
             acc[a][b] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(fa[a], fb[b], acc[a][b], 0,
                                                                         0, 0, 0, 0, 0);
       }
@@ -155,16 +168,16 @@ __global__ void __launch_bounds__(GT_THREADS) gemm_tn_kernel(GemmArgs g) {
 
   float *C = g.C + (long)blockIdx.z * g.split_stride;
 #pragma unroll
-  for (int b = 0; b < 2; b++) {
-    const int c = n0 + wn * 64 + b * 32 + r32;
+  for (int b = 0; b < WN; b++) {
+    const int c = n0 + wn * 32 * WN + b * 32 + r32;
     if (c >= ncols) continue;
     const int phys = g.col_map ? g.col_map[c] : c;
     const float cs = g.col_scale ? g.col_scale[phys] : 1.0f;
 #pragma unroll
-    for (int a = 0; a < 2; a++)
+    for (int a = 0; a < WM; a++)
 #pragma unroll
       for (int q = 0; q < 4; q++) {
-        const int r = m0 + wm * 64 + a * 32 + q * 8 + h * 4;
+        const int r = m0 + wm * 32 * WM + a * 32 + q * 8 + h * 4;
         float4 v = make_float4(acc[a][b][4 * q], acc[a][b][4 * q + 1], acc[a][b][4 * q + 2],
                                acc[a][b][4 * q + 3]);
         if (g.row_scale) {

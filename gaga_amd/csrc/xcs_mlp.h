@@ -474,7 +474,7 @@ static int mlp_pass(xcs_mlp *net, int B, bool train, float pos_w) {
   ga.K = (int)H;
   ga.N = B;
   ga.ksplit = 1;
-  hipLaunchKernelGGL(gemm_tn_kernel<FMT_BF16>, dim3(H / GT_M, (B + GT_N - 1) / GT_N, 1),
+  hipLaunchKernelGGL((gemm_tn_kernel<FMT_BF16, 2, 2>), dim3(H / GT_M, (B + GT_N - 1) / GT_N, 1),
                      dim3(GT_THREADS), 0, st, ga);
   hipLaunchKernelGGL(mlp_l3_kernel, dim3((unsigned)((B * 64L + 255) / 256)), dim3(256), 0, st,
                      net->z2, B, net->p, train ? net->lab : nullptr, pos_w, net->logit, net->loss,
@@ -513,7 +513,7 @@ static int mlp_pass(xcs_mlp *net, int B, bool train, float pos_w) {
   gb.K = (int)H;
   gb.N = B;
   gb.ksplit = 1;
-  hipLaunchKernelGGL(gemm_tn_kernel<FMT_BF16>, dim3(H / GT_M, (B + GT_N - 1) / GT_N, 1),
+  hipLaunchKernelGGL((gemm_tn_kernel<FMT_BF16, 2, 2>), dim3(H / GT_M, (B + GT_N - 1) / GT_N, 1),
                      dim3(GT_THREADS), 0, st, gb);
   hipLaunchKernelGGL(mlp_relu_back_kernel, dim3((unsigned)((B * H + 255) / 256)), dim3(256), 0, st,
                      net->dh1, net->h1b, (long)B * H);

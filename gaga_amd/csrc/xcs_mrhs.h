@@ -893,6 +893,7 @@ struct xcs_mrhs {
   double *rowpart = nullptr; // exact-residual row sums per slice (count x MR_SLICES x m)
   int nblk = 1;
   int polish_max = 6;  // support-correction steps
+  int gemm_big = 0;    // 1: 256 x 128 GEMM block tile where M allows (measured slower)
   xcs_options opts{};
   float phase_ms[6] = {0, 0, 0, 0, 0, 0};  // init, low precision, bf16 continuation, refinement, corrections, spare
   ~xcs_mrhs() {
@@ -945,11 +946,19 @@ static int mr_gemm(xcs_mrhs *h, int fmt, bool transpose, const uint8_t *B, float
   g.ncols = ncols;
   g.col_map = map;
   g.col_scale = fmt == FMT_FP8 ? bscale : nullptr;
-  const dim3 grid(g.M / GT_M, (h->count + GT_N - 1) / GT_N, g.ksplit);
-  if (fmt == FMT_BF16)
-    hipLaunchKernelGGL(gemm_tn_kernel<FMT_BF16>, grid, dim3(GT_THREADS), 0, h->stream, g);
-  else
-    hipLaunchKernelGGL(gemm_tn_kernel<FMT_FP8>, grid, dim3(GT_THREADS), 0, h->stream, g);
+  const bool big = h->gemm_big && g.M % 256 == 0;  // 256 x 128 block tile
+  const dim3 grid(g.M / (big ? 256 : GT_M), (h->count + GT_N - 1) / GT_N, g.ksplit);
+  if (fmt == FMT_BF16) {
+    if (big)
+      hipLaunchKernelGGL((gemm_tn_kernel<FMT_BF16, 4, 2>), grid, dim3(GT_THREADS), 0, h->stream, g);
+    else
+      hipLaunchKernelGGL((gemm_tn_kernel<FMT_BF16, 2, 2>), grid, dim3(GT_THREADS), 0, h->stream, g);
+  } else {
+    if (big)
+      hipLaunchKernelGGL((gemm_tn_kernel<FMT_FP8, 4, 2>), grid, dim3(GT_THREADS), 0, h->stream, g);
+    else
+      hipLaunchKernelGGL((gemm_tn_kernel<FMT_FP8, 2, 2>), grid, dim3(GT_THREADS), 0, h->stream, g);
+  }
   HIP_TRY(hipGetLastError());
   if (g.ksplit > 1) {
     hipLaunchKernelGGL(gemm_split_sum_kernel, dim3(2048), dim3(256), 0, h->stream, C, g.ldc,
@@ -1340,6 +1349,12 @@ int xcs_mrhs_configure(xcs_mrhs *h, float c_bf16, float c_fp8, int exact_max, in
   return XCS_OK;
 }
 
+int xcs_mrhs_gemm_tile(xcs_mrhs *h, int big) {
+  if (!h) return set_err(XCS_EINVAL, "NULL handle");
+  h->gemm_big = big ? 1 : 0;
+  return XCS_OK;
+}
+
 int xcs_mrhs_phase_ms(xcs_mrhs *h, float *ms6) {
   if (!h || !ms6) return set_err(XCS_EINVAL, "NULL argument");
   for (int e = 0; e < 6; e++) ms6[e] = h->phase_ms[e];
@@ -1392,11 +1407,19 @@ int xcs_gemm_tn(const char *fmt, int M, int N, int K, const float *A, const floa
   g.ksplit = 1;
   g.row_scale = f == FMT_FP8 ? rs : nullptr;
   g.col_scale = f == FMT_FP8 ? csb : nullptr;
-  const dim3 grid(M / GT_M, (N + GT_N - 1) / GT_N, 1);
-  if (f == FMT_BF16)
-    hipLaunchKernelGGL(gemm_tn_kernel<FMT_BF16>, grid, dim3(GT_THREADS), 0, h.stream, g);
-  else
-    hipLaunchKernelGGL(gemm_tn_kernel<FMT_FP8>, grid, dim3(GT_THREADS), 0, h.stream, g);
+  const bool big = M % 256 == 0;  // exercise both block tiles
+  const dim3 grid(M / (big ? 256 : GT_M), (N + GT_N - 1) / GT_N, 1);
+  if (f == FMT_BF16) {
+    if (big)
+      hipLaunchKernelGGL((gemm_tn_kernel<FMT_BF16, 4, 2>), grid, dim3(GT_THREADS), 0, h.stream, g);
+    else
+      hipLaunchKernelGGL((gemm_tn_kernel<FMT_BF16, 2, 2>), grid, dim3(GT_THREADS), 0, h.stream, g);
+  } else {
+    if (big)
+      hipLaunchKernelGGL((gemm_tn_kernel<FMT_FP8, 4, 2>), grid, dim3(GT_THREADS), 0, h.stream, g);
+    else
+      hipLaunchKernelGGL((gemm_tn_kernel<FMT_FP8, 2, 2>), grid, dim3(GT_THREADS), 0, h.stream, g);
+  }
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemcpyAsync(C, dC, 4 * (size_t)M * N, hipMemcpyDeviceToHost, h.stream));
   if (A_q) {

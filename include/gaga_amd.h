@@ -119,15 +119,17 @@ int xcs_batch_create(const char *alg, int count, const int *ens, const int *k, c
  * With one workgroup per instance the reset and the launch are a captured HIP graph,
  * replayed on every call. */
 int xcs_batch_solve(xcs_batch *batch, float *ms);
-/* Change the team geometry (workgroups per instance, threads per workgroup); results
- * do not depend on it. */
-int xcs_batch_configure(xcs_batch *batch, int team_size, int block_size);
-/* Autotuner: time every candidate geometry (team 1/2/4/8 x block 256/512/1024) on this
- * batch with each instance capped at probe_iters iterations (0: whole solves), keep the
- * fastest.  cache_path (may be NULL) is a text file of "<signature> <team> <block> <ms>"
- * lines: a hit skips the probes, a miss appends the result. */
-int xcs_batch_tune(xcs_batch *batch, int probe_iters, const char *cache_path, int *team_size,
-                   int *block_size, float *probe_ms);
+/* Change the team geometry: team_size workgroups of block_size threads per instance for the
+ * big_count largest instances (big_count < 0: all of them); the rest run one per workgroup
+ * once those are taken.  Recovery does not depend on it (reduction order does). */
+int xcs_batch_configure(xcs_batch *batch, int team_size, int block_size, int big_count);
+/* Autotuner: time every candidate geometry (team 1/2/4/8 x block 256/512/1024 x teams for all,
+ * or for the largest 1/16, 1/8, 1/4 of the instances) on this batch with each instance capped
+ * at probe_iters iterations (0: whole solves), keep the fastest.  cache_path (may be NULL) is
+ * a text file of "<signature> <team> <block> <big_count> <ms>" lines: a hit skips the probes,
+ * a miss appends the result.  verbose != 0 prints each candidate's time to stderr. */
+int xcs_batch_tune(xcs_batch *batch, int probe_iters, const char *cache_path, int verbose,
+                   int *team_size, int *block_size, int *big_count, float *probe_ms);
 /* Per-instance results, caller indexing: norms[3*i..3*i+2] relative l1/l2/linf. */
 int xcs_batch_results(xcs_batch *batch, float *norms, int *iterations, int *stop_reason,
                       float *conv_rate);
@@ -180,6 +182,9 @@ int xcs_mrhs_fp32_solve(xcs_mrhs *handle, int first, int cnt, int mode, float *m
  * (CG) steps (2, 4). */
 int xcs_mrhs_configure(xcs_mrhs *handle, float c_bf16, float c_fp8, int corrections,
                        int ref_outer, int ref_inner);
+/* GEMM block tile of the products: big = 0 (default) 128 x 128; 1: 256 x 128 where the
+ * row count allows (measured 15-30% slower at 4096 x 16384 x 512). */
+int xcs_mrhs_gemm_tile(xcs_mrhs *handle, int big);
 /* Device ms of the last solve's phases: start, low-precision phase, BF16 continuation,
  * refinement, support corrections (with their refinements), 0. */
 int xcs_mrhs_phase_ms(xcs_mrhs *handle, float *ms6);

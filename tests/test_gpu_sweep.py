@@ -57,14 +57,15 @@ def test_grid_matches_single_instances_and_replays():
     B.close()
 
 
-@pytest.mark.parametrize("team,block", [(2, 512), (4, 1024), (1, 1024)])
-def test_geometries_recover_the_same_instances(team, block):
+@pytest.mark.parametrize("team,block,big", [(2, 512, -1), (4, 1024, -1), (1, 1024, -1),
+                                            (4, 1024, 9), (2, 256, 30)])
+def test_geometries_recover_the_same_instances(team, block, big):
     import gaga_amd as ga
     specs = _small_grid()
     B = ga.Batch("CGIHT", specs, _opts(ga))
     B.solve()
     base = B.results()
-    B.configure(team, block)
+    B.configure(team, block, big)
     B.solve()
     r = B.results()
     ok0 = base["norms"][:, 2] < 1e-3
@@ -83,17 +84,17 @@ def test_tuner_and_cache(tmp_path):
     specs = _small_grid()
     cache = str(tmp_path / "tune.txt")
     B = ga.Batch("CGIHT", specs, _opts(ga))
-    c, b, ms = B.tune(probe_iters=8, cache=cache)
+    c, b, big, ms = B.tune(probe_iters=8, cache=cache)
     assert c in (1, 2, 4, 8) and b in (256, 512, 1024) and ms > 0
     assert B.info()["team_size"] == c and B.info()["block"] == b
     B.solve()
     ok = B.results()["norms"][:, 2] < 1e-3
     B.close()
     lines = open(cache).read().strip().splitlines()
-    assert len(lines) == 1 and lines[0].split()[1:3] == [str(c), str(b)]
+    assert len(lines) == 1 and lines[0].split()[1:4] == [str(c), str(b), str(big)]
     B2 = ga.Batch("CGIHT", specs, _opts(ga))
-    c2, b2, ms2 = B2.tune(probe_iters=8, cache=cache)  # cache hit: no probes
-    assert (c2, b2) == (c, b) and abs(ms2 - ms) < 1e-3
+    c2, b2, big2, ms2 = B2.tune(probe_iters=8, cache=cache)  # cache hit: no probes
+    assert (c2, b2, big2) == (c, b, big) and abs(ms2 - ms) < 1e-3
     B2.solve()
     ok2 = B2.results()["norms"][:, 2] < 1e-3
     np.testing.assert_array_equal(ok, ok2)
