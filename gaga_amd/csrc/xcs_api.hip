@@ -88,6 +88,9 @@ __device__ __forceinline__ int batch_team_leader(int c) {
 // and its workgroups take [n_big, count) one per workgroup from queue[1].  A
 // team works in its leader's workspace and a lone workgroup in its own, so a
 // dissolved workgroup never shares scratch with a team still running.
+// TEAMS = false is the one-workgroup-per-instance form: the team queue is
+// compiled out, which keeps its registers (and spills) out of the kernel.
+template <bool TEAMS>
 __global__ void __launch_bounds__(1024) solve_batch_kernel(const XProblem *Ps, int count,
                                                            int n_big, int *queue,
                                                            const XWork *works, int *slots,
@@ -96,7 +99,7 @@ __global__ void __launch_bounds__(1024) solve_batch_kernel(const XProblem *Ps, i
   __shared__ Lds L;
   __shared__ XProblem sP;
   __shared__ int sidx;
-  if (c > 1) {
+  if (TEAMS && c > 1) {
     int team_id, rank;
     batch_team(c, &team_id, &rank);
     const int leader = batch_team_leader(c);
@@ -819,8 +822,13 @@ static int batch_occupancy(int block, int *per_cu, int *cus) {
   int dev;
   HIP_TRY(hipGetDevice(&dev));
   HIP_TRY(hipDeviceGetAttribute(cus, hipDeviceAttributeMultiprocessorCount, dev));
-  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, (const void *)solve_batch_kernel,
+  // both forms must fit co-resident at the chosen grid size
+  int a = 0, b = 0;
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, (const void *)solve_batch_kernel<false>,
                                                        block, 0));
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, (const void *)solve_batch_kernel<true>,
+                                                       block, 0));
+  *per_cu = std::min(a, b);
   if (*per_cu <= 0) return set_err(XCS_EINVAL, "block size %d does not fit on a CU", block);
   return XCS_OK;
 }
@@ -1122,11 +1130,11 @@ static int batch_enqueue(xcs_batch *B) {
                   (void *)&B->dW, (void *)&B->slots, (void *)&B->red, (void *)&B->bar,
                   (void *)&B->c, (void *)&B->budget};
   if (B->c > 1) {
-    HIP_TRY(hipLaunchCooperativeKernel((const void *)solve_batch_kernel, dim3(B->grid),
+    HIP_TRY(hipLaunchCooperativeKernel((const void *)solve_batch_kernel<true>, dim3(B->grid),
                                        dim3(B->block), args, 0, B->stream));
   } else {
-    HIP_TRY(hipLaunchKernel((const void *)solve_batch_kernel, dim3(B->grid), dim3(B->block), args,
-                            0, B->stream));
+    HIP_TRY(hipLaunchKernel((const void *)solve_batch_kernel<false>, dim3(B->grid),
+                            dim3(B->block), args, 0, B->stream));
   }
   return XCS_OK;
 }
