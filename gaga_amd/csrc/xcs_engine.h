@@ -2,8 +2,6 @@
 #pragma once
 #include <stdint.h>
 
-#include <type_traits>
-
 namespace xcs {
 
 // Host-side assignment of a plain pointer to a descriptor field.
@@ -62,33 +60,6 @@ struct XProblem {
   XG(const int) warm;         // optional CGIHT warm start: support mask (n), 1 = in
   int warm_cg;                // CG steps of the least-squares start on that support
 };
-
-// Epilogues of the A^T products run once per column after its dot product.
-// An epilogue that takes a third argument receives bins[j] and gprev[j]
-// loaded before the column's stream starts, so the wave does not stall on a
-// dependent load after the dot product; two-argument epilogues load nothing.
-struct EpiPre {
-  int bin;
-  float gp;
-};
-template <class Epi>
-constexpr bool kWantsPre = std::is_invocable_v<Epi &, int, float, const EpiPre &>;
-template <class Epi>
-__device__ __forceinline__ EpiPre epi_prefetch(const XProblem &P, int j) {
-  EpiPre p{0, 0.f};
-  if constexpr (kWantsPre<Epi>) {
-    p.bin = P.bins[j];
-    p.gp = P.gprev[j];
-  }
-  return p;
-}
-template <class Epi>
-__device__ __forceinline__ void call_epi(Epi &epi, int j, float g, const EpiPre &p) {
-  if constexpr (kWantsPre<Epi>)
-    epi(j, g, p);
-  else
-    epi(j, g);
-}
 
 constexpr int MAXCH = 64;  // gen gather: max column chunks (partials rows)
 

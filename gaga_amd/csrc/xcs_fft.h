@@ -175,8 +175,8 @@ __device__ void dct_AT(Team &t, Lds &L, const XProblem &P, XG(const float) r, Ep
   t.sync();
   team_fft(t, L, z, mid, Y, P.twiddle, n, P.log2n, +1);
   for (int q = t.gtid(); q < n / 2; q += t.gsize()) {
-    call_epi(epi, 2 * q, Y[q].x, epi_prefetch<Epi>(P, 2 * q));
-    call_epi(epi, 2 * q + 1, Y[n - 1 - q].x, epi_prefetch<Epi>(P, 2 * q + 1));
+    epi(2 * q, Y[q].x);
+    epi(2 * q + 1, Y[n - 1 - q].x);
   }
 }
 

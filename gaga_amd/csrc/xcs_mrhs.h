@@ -24,7 +24,9 @@
 //   corrections for signals still above tol: g = A^T r (exact r), the NIHT
 //               step on the current support, x += mu g, and the refinement
 //               again, which keeps the k largest entries (hard thresholding
-//               pursuit with an exact k-term threshold).
+//               pursuit with an exact k-term threshold).  Every product and
+//               per-signal kernel of a correction runs over those signals
+//               only; the others keep their refined solution.
 // The FP64 residual of the FP32 A decides success against tol, as the
 // reference's stopping rule does.
 
@@ -492,9 +494,9 @@ __global__ void __launch_bounds__(MR_VT) mr_switch_kernel(MrhsDesc D, float c_lp
 // taken in index order); x is cut to it and bins2 = 1 marks it.  The k-th
 // magnitude is found by a bitwise search on the IEEE bits of |x| (monotone
 // for non-negative floats).
-__global__ void __launch_bounds__(MR_VT) mr_ref_supp_kernel(MrhsDesc D, int topk) {
+__global__ void __launch_bounds__(MR_VT) mr_ref_supp_kernel(MrhsDesc D, int topk, int sel) {
   const int s = blockIdx.x;
-  if (!topk && D.stage[s] != 0) return;
+  if ((!topk || sel) && D.stage[s] != 0) return;
   const long n = D.n;
   float *x = D.x + s * n;
   int *mask = D.bins2 + s * n, *sup = D.supp + s * n;
@@ -562,10 +564,12 @@ __global__ void __launch_bounds__(MR_VT) mr_ref_supp_kernel(MrhsDesc D, int topk
 constexpr int MR_SLICES = 64;
 constexpr int MR_CHUNK = 16;
 constexpr int MR_MAXQ = 8;
-__global__ void __launch_bounds__(MR_THREADS) mr_exact_resid_kernel(MrhsDesc D, double *rowpart) {
+__global__ void __launch_bounds__(MR_THREADS) mr_exact_resid_kernel(MrhsDesc D, double *rowpart,
+                                                                    int sel) {
   __shared__ int sj[MR_CHUNK];
   __shared__ float sv[MR_CHUNK];
   const int z = blockIdx.x, s = blockIdx.y;
+  if (sel && D.stage[s] != 0) return;
   const int slices = MR_SLICES;
   const long n = D.n, m = D.m;
   const int cnt = D.supp_cnt[s];
@@ -623,9 +627,10 @@ __global__ void __launch_bounds__(MR_THREADS) mr_exact_resid_kernel(MrhsDesc D, 
 
 __global__ void __launch_bounds__(MR_THREADS) mr_resid_sum_kernel(MrhsDesc D,
                                                                   const double *rowpart,
-                                                                  double *part) {
+                                                                  double *part, int sel) {
   __shared__ double red[MR_THREADS / 64];
   const int s = blockIdx.y, slices = MR_SLICES;
+  if (sel && D.stage[s] != 0) return;
   const long m = D.m;
   const int cnt = D.supp_cnt[s];
   const int used = min(slices, (cnt + MR_CHUNK - 1) / MR_CHUNK);  // slices that wrote
@@ -657,8 +662,9 @@ __device__ __forceinline__ double mr_part_sum(const double *part, int s, int nbl
 // Refinement residual (after mr_exact_resid_kernel): ||r||^2 into
 // ref; d = 0; r as the next A^T operand (BF16).
 __global__ void __launch_bounds__(MR_VT) mr_ref_resid_kernel(MrhsDesc D, const double *part,
-                                                                  int nblk) {
+                                                                  int nblk, int sel) {
   const int s = blockIdx.x;
+  if (sel && D.stage[s] != 0) return;
   const long n = D.n, m = D.m;
   float *d = D.thres + s * n;
   for (int j = threadIdx.x; j < D.n; j += blockDim.x) d[j] = 0.f;
@@ -669,16 +675,20 @@ __global__ void __launch_bounds__(MR_VT) mr_ref_resid_kernel(MrhsDesc D, const d
 }
 
 // Support correction on the exact residual (after a refinement, whose final
-// pass left r in resid and its BF16 copy in rb, and g = A^T r in D.g): the
-// signals with ||r|| > tol are active (stage 0); g restricted to the current
-// support (bins2) is the operand of the step-size product.
+// pass left r in resid and its BF16 copy in rb): mr_htp_mark_kernel makes the
+// signals with ||r|| > tol active (stage 0); after g = A^T r over them, g
+// restricted to the current support (bins2) is the operand of the step-size
+// product.
+__global__ void __launch_bounds__(MR_THREADS) mr_htp_mark_kernel(MrhsDesc D) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= D.count) return;
+  const double rn = sqrt(D.ref[4 * s + 1]);
+  D.stage[s] = (D.st[s].max_value != 0.f && rn > (double)D.tol) ? 0 : 1;
+}
 __global__ void __launch_bounds__(MR_VT) mr_htp_grad_kernel(MrhsDesc D) {
   const int s = blockIdx.x;
   const long n = D.n;
-  const double rn = sqrt(D.ref[4 * s + 1]);
-  const int act = (D.st[s].max_value != 0.f && rn > (double)D.tol) ? 1 : 0;
-  if (threadIdx.x == 0) D.stage[s] = act ? 0 : 1;
-  if (!act) return;
+  if (D.stage[s] != 0) return;
   const float *G = D.g + s * n;
   const int *mask = D.bins2 + s * n;
   float *gs = D.grad + s * n;
@@ -718,8 +728,9 @@ __global__ void __launch_bounds__(MR_VT) mr_htp_step_kernel(MrhsDesc D) {
 
 // CG on A_S^T A_S d = A_S^T r, part 1 (after s = A^T r): gamma = ||s_S||^2,
 // p = s_S + beta p (beta = 0 on the first step); p as the next A operand.
-__global__ void __launch_bounds__(MR_VT) mr_ref_dir_kernel(MrhsDesc D, int first) {
+__global__ void __launch_bounds__(MR_VT) mr_ref_dir_kernel(MrhsDesc D, int first, int sel) {
   const int s = blockIdx.x;
+  if (sel && D.stage[s] != 0) return;
   const long n = D.n;
   const float *G = D.g + s * n;
   const int *mask = D.bins2 + s * n;
@@ -744,8 +755,9 @@ __global__ void __launch_bounds__(MR_VT) mr_ref_dir_kernel(MrhsDesc D, int first
 
 // CG part 2 (after q = A p): alpha = gamma / ||q||^2, d += alpha p,
 // r -= alpha q; r as the next A^T operand.
-__global__ void __launch_bounds__(MR_VT) mr_ref_step_kernel(MrhsDesc D) {
+__global__ void __launch_bounds__(MR_VT) mr_ref_step_kernel(MrhsDesc D, int sel) {
   const int s = blockIdx.x;
+  if (sel && D.stage[s] != 0) return;
   const long n = D.n, m = D.m;
   const float *q = D.w1 + s * m, *p = D.gprev_thres + s * n;
   float *r = D.resid + s * m, *d = D.thres + s * n;
@@ -764,8 +776,9 @@ __global__ void __launch_bounds__(MR_VT) mr_ref_step_kernel(MrhsDesc D) {
 }
 
 // x_S += d.
-__global__ void __launch_bounds__(MR_VT) mr_ref_update_kernel(MrhsDesc D) {
+__global__ void __launch_bounds__(MR_VT) mr_ref_update_kernel(MrhsDesc D, int sel) {
   const int s = blockIdx.x;
+  if (sel && D.stage[s] != 0) return;
   const long n = D.n;
   float *x = D.x + s * n;
   const float *d = D.thres + s * n;
@@ -1097,11 +1110,11 @@ static int mr_create(xcs_mrhs *h, int m, int n, int k, int count, const char *pr
   return XCS_OK;
 }
 
-static int mr_exact_resid(xcs_mrhs *h) {
+static int mr_exact_resid(xcs_mrhs *h, int sel) {
   hipLaunchKernelGGL(mr_exact_resid_kernel, dim3(MR_SLICES, h->count), dim3(MR_THREADS), 0,
-                     h->stream, h->D, h->rowpart);
+                     h->stream, h->D, h->rowpart, sel);
   hipLaunchKernelGGL(mr_resid_sum_kernel, dim3(h->nblk, h->count), dim3(MR_THREADS), 0, h->stream,
-                     h->D, h->rowpart, h->part);
+                     h->D, h->rowpart, h->part, sel);
   HIP_TRY(hipGetLastError());
   return XCS_OK;
 }
@@ -1140,28 +1153,33 @@ static int mr_phase(xcs_mrhs *h, int *issued) {
   return XCS_OK;
 }
 
-static int mr_refine(xcs_mrhs *h) {
+// sel = 0: every signal; sel = 1: the active signals only (stage 0, listed in
+// D.active).  Signals are independent, and each column of a GEMM is the same
+// K-ordered sum whichever columns share its tile, so a selected signal gets
+// the same bits either way.
+static int mr_refine(xcs_mrhs *h, int sel) {
   MrhsDesc &D = h->D;
   D.fmt = FMT_BF16;
   const dim3 gs(h->count), bs(MR_VT);
-  hipLaunchKernelGGL(mr_ref_supp_kernel, gs, bs, 0, h->stream, D, 1);
+  const int *nc = sel ? D.nactive : nullptr, *map = sel ? D.active : nullptr;
+  hipLaunchKernelGGL(mr_ref_supp_kernel, gs, bs, 0, h->stream, D, 1, sel);
   for (int o = 0; o < h->ref_outer; o++) {
-    XCS_TRY(mr_exact_resid(h));
-    hipLaunchKernelGGL(mr_ref_resid_kernel, gs, bs, 0, h->stream, D, h->part, h->nblk);
-    XCS_TRY(mr_gemm(h, FMT_BF16, true, D.rb, D.g, nullptr, nullptr, nullptr));
-    hipLaunchKernelGGL(mr_ref_dir_kernel, gs, bs, 0, h->stream, D, 1);
+    XCS_TRY(mr_exact_resid(h, sel));
+    hipLaunchKernelGGL(mr_ref_resid_kernel, gs, bs, 0, h->stream, D, h->part, h->nblk, sel);
+    XCS_TRY(mr_gemm(h, FMT_BF16, true, D.rb, D.g, nullptr, nc, map));
+    hipLaunchKernelGGL(mr_ref_dir_kernel, gs, bs, 0, h->stream, D, 1, sel);
     for (int j = 0; j < h->ref_inner; j++) {
-      XCS_TRY(mr_gemm(h, FMT_BF16, false, D.vb1, D.w1, nullptr, nullptr, nullptr));
-      hipLaunchKernelGGL(mr_ref_step_kernel, gs, bs, 0, h->stream, D);
+      XCS_TRY(mr_gemm(h, FMT_BF16, false, D.vb1, D.w1, nullptr, nc, map));
+      hipLaunchKernelGGL(mr_ref_step_kernel, gs, bs, 0, h->stream, D, sel);
       if (j + 1 == h->ref_inner) break;
-      XCS_TRY(mr_gemm(h, FMT_BF16, true, D.rb, D.g, nullptr, nullptr, nullptr));
-      hipLaunchKernelGGL(mr_ref_dir_kernel, gs, bs, 0, h->stream, D, 0);
+      XCS_TRY(mr_gemm(h, FMT_BF16, true, D.rb, D.g, nullptr, nc, map));
+      hipLaunchKernelGGL(mr_ref_dir_kernel, gs, bs, 0, h->stream, D, 0, sel);
     }
-    hipLaunchKernelGGL(mr_ref_update_kernel, gs, bs, 0, h->stream, D);
+    hipLaunchKernelGGL(mr_ref_update_kernel, gs, bs, 0, h->stream, D, sel);
   }
   // final FP64 residual
-  XCS_TRY(mr_exact_resid(h));
-  hipLaunchKernelGGL(mr_ref_resid_kernel, gs, bs, 0, h->stream, D, h->part, h->nblk);
+  XCS_TRY(mr_exact_resid(h, sel));
+  hipLaunchKernelGGL(mr_ref_resid_kernel, gs, bs, 0, h->stream, D, h->part, h->nblk, sel);
   HIP_TRY(hipGetLastError());
   return XCS_OK;
 }
@@ -1193,22 +1211,24 @@ static int mr_solve(xcs_mrhs *h, float *ms) {
     XCS_TRY(mr_phase(h, &h->lp_iters[1]));
   }
   HIP_TRY(hipEventRecord(ev[3], h->stream));
-  XCS_TRY(mr_refine(h));
+  XCS_TRY(mr_refine(h, 0));
   HIP_TRY(hipEventRecord(ev[4], h->stream));
   // signals above tol after the refinement: support corrections on the
   // exact residual, each followed by the refinement
   D.fmt = FMT_BF16;
   for (int c = 0; c < h->polish_max; c++) {
-    XCS_TRY(mr_gemm(h, FMT_BF16, true, D.rb, D.g, nullptr, nullptr, nullptr));
-    hipLaunchKernelGGL(mr_htp_grad_kernel, gs, bs, 0, h->stream, D);
+    hipLaunchKernelGGL(mr_htp_mark_kernel, dim3((h->count + MR_THREADS - 1) / MR_THREADS),
+                       dim3(MR_THREADS), 0, h->stream, D);
     hipLaunchKernelGGL(mr_list_kernel, dim3(1), dim3(MR_THREADS), 0, h->stream, D);
     HIP_TRY(hipMemcpyAsync(h->h_nactive, D.nactive, sizeof(int), hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
     if (*h->h_nactive == 0) break;
+    XCS_TRY(mr_gemm(h, FMT_BF16, true, D.rb, D.g, nullptr, D.nactive, D.active));
+    hipLaunchKernelGGL(mr_htp_grad_kernel, gs, bs, 0, h->stream, D);
     XCS_TRY(mr_gemm(h, FMT_BF16, false, D.vb1, D.w1, nullptr, D.nactive, D.active));
     hipLaunchKernelGGL(mr_htp_step_kernel, gs, bs, 0, h->stream, D);
     HIP_TRY(hipGetLastError());
-    XCS_TRY(mr_refine(h));
+    XCS_TRY(mr_refine(h, 1));
     h->lp_iters[2]++;
   }
   HIP_TRY(hipEventRecord(ev[5], h->stream));

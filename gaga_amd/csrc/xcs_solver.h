@@ -161,8 +161,6 @@ __device__ void gen_AT(Team &t, Lds &L, const XProblem &P, XG(const float) r, Ep
   constexpr int U = 4;
   for (int j0 = 2 * gw; j0 < n; j0 += 2 * GW) {
     const bool two = j0 + 1 < n;
-    const EpiPre p0 = epi_prefetch<Epi>(P, j0);
-    const EpiPre p1 = epi_prefetch<Epi>(P, two ? j0 + 1 : j0);
     XG(const float4) c0 = reinterpret_cast<XG(const float4)>(A + (size_t)j0 * lda);
     XG(const float4) c1 = reinterpret_cast<XG(const float4)>(A + (size_t)(two ? j0 + 1 : j0) * lda);
     float s00 = 0.f, s01 = 0.f, s10 = 0.f, s11 = 0.f;
@@ -198,8 +196,8 @@ __device__ void gen_AT(Team &t, Lds &L, const XProblem &P, XG(const float) r, Ep
     const float g0 = wave_sum(s00 + s01);
     const float g1 = wave_sum(s10 + s11);
     if (lane == 0) {
-      call_epi(epi, j0, g0, p0);
-      if (two) call_epi(epi, j0 + 1, g1, p1);
+      epi(j0, g0);
+      if (two) epi(j0 + 1, g1);
     }
   }
   if (lds) __syncthreads();
@@ -223,7 +221,7 @@ __device__ void smv_AT(Team &t, const XProblem &P, XG(const float) r, Epi &&epi)
     float s = 0.f;
     const int e1 = P.col_ptr[j + 1];
     for (int e = P.col_ptr[j]; e < e1; e++) s = fmaf(P.vals[e], r[P.rows[e]], s);
-    call_epi(epi, j, s, epi_prefetch<Epi>(P, j));
+    epi(j, s);
   }
 }
 
@@ -426,9 +424,9 @@ __device__ float step_restricted_sd(Team &t, Lds &L, const XProblem &P, XState &
     push16(S.prev, (float)sqrt(v[0]));
   }
   float gmax = 0.f, th2 = 0.f;
-  opAT(t, L, P, S, P.resid, [&](int j, float g, const EpiPre &p) {
+  opAT(t, L, P, S, P.resid, [&](int j, float g) {
     P.grad[j] = g;
-    const float th = (p.bin <= kb) ? g : 0.f;
+    const float th = (P.bins[j] <= kb) ? g : 0.f;
     P.thres[j] = th;
     th2 = fmaf(th, th, th2);
     gmax = fmaxf(gmax, fabsf(g));
@@ -479,9 +477,9 @@ __device__ void step_restricted_cg(Team &t, Lds &L, const XProblem &P, XState &S
     alpha = 0.f;
   for (int i = t.gtid(); i < P.n; i += t.gsize()) P.x[i] = fmaf(alpha, P.grad[i], P.x[i]);
   float gp2 = 0.f;
-  opAT(t, L, P, S, P.resid, [&](int j, float g, const EpiPre &p) {
-    const float th = (p.bin <= kb) ? g : 0.f;
-    const float gp = fmaf(-alpha, th, p.gp);
+  opAT(t, L, P, S, P.resid, [&](int j, float g) {
+    const float th = (P.bins[j] <= kb) ? g : 0.f;
+    const float gp = fmaf(-alpha, th, P.gprev[j]);
     P.gprev[j] = gp;
     gp2 = fmaf(gp, gp, gp2);
   });
@@ -524,8 +522,8 @@ __device__ float cg_projection(Team &t, Lds &L, const XProblem &P, XState &S, bo
   double e2 = residual(t, L, P, S, P.resid, P.resid_update);
   t.sync();
   float g2 = 0.f;
-  opAT(t, L, P, S, P.resid, [&](int j, float g, const EpiPre &p) {
-    const float th = (p.bin <= kb) ? g : 0.f;
+  opAT(t, L, P, S, P.resid, [&](int j, float g) {
+    const float th = (P.bins[j] <= kb) ? g : 0.f;
     P.grad[j] = th;
     P.gprev[j] = th;
     g2 = fmaf(th, th, g2);
@@ -563,9 +561,9 @@ __device__ float step_cg_restricted(Team &t, Lds &L, const XProblem &P, XState &
   }
   PHASE(P, t, 1);
   float th2 = 0.f;
-  opAT(t, L, P, S, P.resid, [&](int j, float g, const EpiPre &p) {
+  opAT(t, L, P, S, P.resid, [&](int j, float g) {
     P.grad[j] = g;
-    const float th = (p.bin <= kb) ? g : 0.f;
+    const float th = (P.bins[j] <= kb) ? g : 0.f;
     P.thres[j] = th;
     th2 = fmaf(th, th, th2);
   });
@@ -640,9 +638,9 @@ __device__ float step_cg_unrestricted(Team &t, Lds &L, const XProblem &P, XState
     push16(S.prev, (float)sqrt(v[0]));
   }
   PHASE(P, t, 6);
-  opAT(t, L, P, S, P.resid_update, [&](int j, float g, const EpiPre &p) {
+  opAT(t, L, P, S, P.resid_update, [&](int j, float g) {
     P.grad[j] = g;
-    P.thres[j] = (p.bin <= kb) ? g : 0.f;
+    P.thres[j] = (P.bins[j] <= kb) ? g : 0.f;
   });
   t.sync();
   PHASE(P, t, 7);
@@ -847,9 +845,9 @@ __device__ void solve_instance(Team &t, Lds &L, const XProblem &P, int budget) {
       reduce(t, w, o1);
       S.err = (float)sqrt(w[0]);
       S.prev[15] = S.err;
-      opAT(t, L, P, S, P.resid, [&](int j, float g, const EpiPre &p) {
+      opAT(t, L, P, S, P.resid, [&](int j, float g) {
         P.gprev[j] = g;
-        P.gprev_thres[j] = (p.bin <= S.k_bin) ? g : 0.f;
+        P.gprev_thres[j] = (P.bins[j] <= S.k_bin) ? g : 0.f;
       });
       for (int j = 0; j < 15; j++) S.evo[j] = S.evo[j + 1];
       S.evo[15] = S.prev[14] - S.prev[15];
