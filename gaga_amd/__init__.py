@@ -17,8 +17,9 @@ import numpy as np
 
 from . import _capi
 from ._capi import Plan, Batch, ManyRHS, MLP, GagaError, gemm_tn, mlp_features, load  # noqa: F401
+from ._capi import results_line  # noqa: F401
 
-__all__ = ["cs", "options", "Plan", "Batch", "ManyRHS", "MLP", "gemm_tn", "mlp_features",
+__all__ = ["cs", "options", "Plan", "Batch", "ManyRHS", "MLP", "gemm_tn", "mlp_features", "results_line",
            "GagaError", "load",
            "ALGORITHMS", "ENSEMBLES"]
 
@@ -27,7 +28,10 @@ ENSEMBLES = ("gen", "smv", "dct")
 
 # gagaOptions names (gagaOptions.m:8-58) -> handler
 _VEC_DIST = {"uniform": 0, "binary": 1, "gaussian": 2}
-_MAT_ENS = {"gen": {"gaussian": 1}, "smv": {"ones_scaled": 1, "binary": 2, "ones": 3}, "dct": {}}
+# names and codes of gaga_gen.cu:307-314 and gaga_smv.cu:387-395: smv 'ones' is all ones
+# scaled by 1/sqrt(p), 'binary' random signs scaled by 1/sqrt(p), 'expander' (the default,
+# gaga_smv.cu:241) all ones unscaled (createMeasurements_smv.cu:178-198)
+_MAT_ENS = {"gen": {"gaussian": 1}, "smv": {"ones": 1, "binary": 2, "expander": 3}, "dct": {}}
 _ACCEPTED_IGNORED = {"threadsPerBlockn", "threadsPerBlockm", "threadsPerBlocknp",
                      "threadsPerBlockBin", "kFixed", "alpha", "supportFlag", "timing",
                      "projFracTol"}
@@ -86,7 +90,7 @@ def _apply_options(alg, ens, n, opts):
         elif name == "convRateNum":
             if int(val) != 16:
                 raise ValueError("convRateNum other than 16 is not supported")
-        elif name in _ACCEPTED_IGNORED:
+        elif name in _ACCEPTED_IGNORED or name == "resultsDir":
             continue
         elif name == "noise":
             if float(val) > 0:
@@ -120,12 +124,15 @@ def cs(alg, ens, *args):
     norms are the relative l1, l2, linf errors (results.cu:444-487); times are
     [solve ms, ms per iteration, total ms]; support is [TP, FP, TN, FN].
     Index arrays of the given-problem forms are 1-based like MATLAB's.
+    Option resultsDir (generate forms): append the reference's results line
+    (functions.cu:34-80) to resultsDir/gpu_data_<ens>YYYYMMDD.txt.
     """
     _check(alg, ens)
     args = list(args)
     opts = None
     if args and isinstance(args[-1], (list, tuple)) and (not args[-1] or isinstance(args[-1][0], tuple)):
         opts = args.pop()
+    results_dir = dict(opts or []).get("resultsDir")
     if ens == "gen":
         if len(args) >= 2 and _is_array(args[1]):
             if len(args) != 3:
@@ -139,7 +146,7 @@ def cs(alg, ens, *args):
             raise ValueError("Either five or six input arguments required.")
         k, m, n = (int(a) for a in args)
         o = _apply_options(alg, ens, n, opts)
-        return _capi.generate(alg, ens, k, m, n, 0, o)
+        return _generate(alg, ens, k, m, n, 0, o, results_dir)
     if ens == "smv":
         if len(args) >= 3 and _is_array(args[2]):
             if len(args) != 5:
@@ -156,7 +163,7 @@ def cs(alg, ens, *args):
             raise ValueError("Either six or seven input arguments required.")
         k, m, n, p = (int(a) for a in args)
         o = _apply_options(alg, ens, n, opts)
-        return _capi.generate(alg, ens, k, m, n, p, o)
+        return _generate(alg, ens, k, m, n, p, o, results_dir)
     # dct
     if len(args) >= 2 and _is_array(args[1]):
         if len(args) != 5:
@@ -169,4 +176,16 @@ def cs(alg, ens, *args):
         raise ValueError("Either five or six input arguments required.")
     k, m, n = (int(a) for a in args)
     o = _apply_options(alg, ens, n, opts)
-    return _capi.generate(alg, ens, k, m, n, 0, o)
+    return _generate(alg, ens, k, m, n, 0, o, results_dir)
+
+
+def _generate(alg, ens, k, m, n, p, o, results_dir):
+    """Generate-and-solve; with the option resultsDir (this build's own: the
+    reference always writes to its working directory) the reference's results
+    line is appended to resultsDir/gpu_data_<ens>YYYYMMDD.txt, as
+    results.cu:509-525 does after every generated solve."""
+    out = _capi.generate(alg, ens, k, m, n, p, o)
+    if results_dir is not None:
+        norms, times, iters, sup, rate, _ = out
+        _capi.results_append(results_dir, alg, ens, k, m, n, p, o, norms, times, iters, sup, rate)
+    return out

@@ -21,7 +21,7 @@ EXPORTS = (
     "xcs_mrhs_samples", "xcs_mrhs_gemm_tile", "xcs_mlp_create", "xcs_mlp_destroy", "xcs_mlp_param_count",
     "xcs_mlp_get_params", "xcs_mlp_set_params", "xcs_mlp_grad", "xcs_mlp_scores",
     "xcs_mlp_set_data", "xcs_mlp_train", "xcs_plan_solve_warm", "xcs_mlp_features", "xcs_stream_copy", "xcs_mrhs_destroy",
-    "xcs_last_error", "xcs_version",
+    "xcs_last_error", "xcs_version", "xcs_results_line", "xcs_results_append",
 )
 
 
@@ -63,6 +63,10 @@ def load():
         L.xcs_cs_gen.argtypes = [_C, _P, _P, _I, _I, _I, _P, _P, _P, _P]
         L.xcs_cs_smv.argtypes = [_C, _P, _P, _P, _P, _I, _I, _I, _I, _P, _P, _P, _P]
         L.xcs_cs_dct.argtypes = [_C, _P, _P, _I, _I, _I, _P, _P, _P, _P]
+        L.xcs_results_line.argtypes = [ctypes.c_char_p, ctypes.c_size_t, _C, _C, _I, _I, _I, _I, _I,
+                                       _I, _P, _P, _I, ctypes.c_float, _P, ctypes.c_uint]
+        L.xcs_results_append.argtypes = [_C, _C, _C, _I, _I, _I, _I, _I, _I, _P, _P, _I,
+                                         ctypes.c_float, _P, ctypes.c_uint]
         L.xcs_plan_create.argtypes = [_C, _C, _I, _I, _I, _I, _P, _P]
         L.xcs_plan_solve.argtypes = [_P, _P]
         L.xcs_plan_results.argtypes = [_P, _P, _P, _P, _P, _P, _P, _P]
@@ -144,6 +148,31 @@ def generate(alg, ens, k, m, n, p, o):
                                _p(times), ctypes.byref(it), _p(sup), ctypes.byref(rate), _p(x),
                                None))
     return norms, times, it.value, sup, rate.value, x
+
+
+def _line_args(alg, ens, k, m, n, p, o, norms, times, iters, sup, rate):
+    norms = np.ascontiguousarray(norms, np.float32)
+    times = np.ascontiguousarray(times, np.float32)
+    sup = np.ascontiguousarray(sup, np.int32)
+    return (alg.encode(), ens.encode(), int(k), int(m), int(n), int(p), int(o.matrix_ensemble),
+            int(o.vec_distribution), _p(norms), _p(times), int(iters), float(rate), _p(sup),
+            int(o.seed) & 0xFFFFFFFF), (norms, times, sup)
+
+
+def results_line(alg, ens, k, m, n, p, o, norms, times, iters, sup, rate):
+    """The reference's per-solve results line (File_output*, functions.cu:34-80)."""
+    args, keep = _line_args(alg, ens, k, m, n, p, o, norms, times, iters, sup, rate)
+    buf = ctypes.create_string_buffer(1024)
+    r = load().xcs_results_line(buf, 1024, *args)
+    if r < 0:
+        _ck(r)
+    return buf.value.decode()
+
+
+def results_append(directory, alg, ens, k, m, n, p, o, norms, times, iters, sup, rate):
+    """Append the results line to <directory>/gpu_data_<ens>YYYYMMDD.txt (results.cu:509-525)."""
+    args, keep = _line_args(alg, ens, k, m, n, p, o, norms, times, iters, sup, rate)
+    _ck(load().xcs_results_append(str(directory).encode(), *args))
 
 
 def solve_gen(alg, y, A, k, o):

@@ -7,6 +7,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <ctime>
 #include <string>
 #include <vector>
 
@@ -1417,6 +1418,81 @@ int xcs_plan_bytes(xcs_plan *pl, double *bytes) {
 }
 
 void xcs_plan_destroy(xcs_plan *pl) { delete pl; }
+
+// The per-solve results line of File_output (dct), File_output_smv and
+// File_output_gen (GAGA/gaga/functions.cu:34-80): keys, order, separators and
+// number formats as the reference prints them, so its processing scripts
+// (CGIHTcode/code/process_{gen,smv,dct}_data.m, a whitespace textscan) read it
+// unchanged.  smv prints band_percentage 0.00 (this build draws no banded
+// supports) and noise_level 0.0000 (no noise), as the reference's noiseless
+// path does (results.cu:350).
+static int results_line(std::string &out, const char *alg, int ens, int k, int m, int n, int p,
+                        int mat_ens, int vec_dist, const float *e, const float *t, int iters,
+                        float rate, const int *sp, unsigned int seed) {
+  const char *tag = ens == ENS_GEN ? "gen" : ens == ENS_SMV ? "smv" : "dct";
+  char b[1024];
+  int w = snprintf(b, sizeof(b), "%s_S_%s output: k %d, m %d, n %d, vecDistribution %d ", alg, tag,
+                   k, m, n, vec_dist);
+  if (ens == ENS_SMV)
+    w += snprintf(b + w, sizeof(b) - w, "nonzeros_per_column %d matrixEnsemble %d ", p, mat_ens);
+  else if (ens == ENS_GEN)
+    w += snprintf(b + w, sizeof(b) - w, "matrixEnsemble %d ", mat_ens);
+  w += snprintf(b + w, sizeof(b) - w,
+                "errorl1 %0.7e errorl2 %0.7e errorlinf %0.7e timeALG %0.7e timeIter %0.7e "
+                "timeTotal %0.7e iterations %d converge_rate %0.7e TruePos %d FalsePos %d "
+                "TrueNeg %d FalseNeg %d ",
+                e[0], e[1], e[2], t[0], t[1], t[2], iters, rate, sp[0], sp[1], sp[2], sp[3]);
+  if (ens == ENS_SMV)
+    w += snprintf(b + w, sizeof(b) - w, "seed %u band_percentage %1.2f noise_level %2.4f \n", seed,
+                  0.0f, 0.0f);
+  else
+    w += snprintf(b + w, sizeof(b) - w, "seed %u \n", seed);
+  if (w <= 0 || w >= (int)sizeof(b)) return set_err(XCS_EINVAL, "results line too long");
+  out.assign(b, (size_t)w);
+  return XCS_OK;
+}
+
+int xcs_results_line(char *buf, size_t buf_len, const char *alg, const char *ens, int k, int m,
+                     int n, int p, int matrix_ensemble, int vec_distribution, const float norms[3],
+                     const float times[3], int iterations, float conv_rate, const int support[4],
+                     unsigned int seed) {
+  const int a = parse_alg(alg), e = parse_ens(ens);
+  if (a < 0 || e < 0) return set_err(XCS_EINVAL, "unknown algorithm or ensemble");
+  if (!buf || !norms || !times || !support) return set_err(XCS_EINVAL, "NULL argument");
+  std::string line;
+  XCS_TRY(results_line(line, alg, e, k, m, n, p, matrix_ensemble, vec_distribution, norms, times,
+                       iterations, conv_rate, support, seed));
+  if (line.size() + 1 > buf_len)
+    return set_err(XCS_EINVAL, "buffer of %zu bytes holds no %zu-byte line", buf_len,
+                   line.size() + 1);
+  memcpy(buf, line.c_str(), line.size() + 1);
+  return (int)line.size();
+}
+
+int xcs_results_append(const char *dir, const char *alg, const char *ens, int k, int m, int n,
+                       int p, int matrix_ensemble, int vec_distribution, const float norms[3],
+                       const float times[3], int iterations, float conv_rate,
+                       const int support[4], unsigned int seed) {
+  const int a = parse_alg(alg), e = parse_ens(ens);
+  if (a < 0 || e < 0) return set_err(XCS_EINVAL, "unknown algorithm or ensemble");
+  if (!norms || !times || !support) return set_err(XCS_EINVAL, "NULL argument");
+  std::string line;
+  XCS_TRY(results_line(line, alg, e, k, m, n, p, matrix_ensemble, vec_distribution, norms, times,
+                       iterations, conv_rate, support, seed));
+  // gpu_data_<ens>YYYYMMDD.txt from the local date (results.cu:509-525)
+  const time_t now = time(nullptr);
+  struct tm tmv;
+  localtime_r(&now, &tmv);
+  char name[64];
+  snprintf(name, sizeof(name), "gpu_data_%s%d%02d%02d.txt", ens, tmv.tm_year + 1900,
+           tmv.tm_mon + 1, tmv.tm_mday);
+  std::string path = (dir && *dir) ? std::string(dir) + "/" + name : std::string(name);
+  FILE *f = fopen(path.c_str(), "a+");
+  if (!f) return set_err(XCS_EINVAL, "cannot open %s for appending", path.c_str());
+  const bool ok = fputs(line.c_str(), f) >= 0;
+  if (fclose(f) != 0 || !ok) return set_err(XCS_EINVAL, "cannot write %s", path.c_str());
+  return XCS_OK;
+}
 
 int xcs_cs_generate(const char *alg, const char *ens, int k, int m, int n, int p,
                     const xcs_options *opts, float norms[3], float times[3], int *iterations,

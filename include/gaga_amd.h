@@ -20,6 +20,7 @@
  */
 #ifndef GAGA_AMD_H
 #define GAGA_AMD_H
+#include <stddef.h>
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -68,6 +69,24 @@ int xcs_options_default(const char *alg, const char *ens, int n, xcs_options *op
 int xcs_cs_generate(const char *alg, const char *ens, int k, int m, int n, int p,
                     const xcs_options *opts, float norms[3], float times[3], int *iterations,
                     int support[4], float *conv_rate, float *x_out, float *x_true);
+
+/* The results line the reference appends after every generated solve:
+ * File_output / File_output_smv / File_output_gen (GAGA/gaga/functions.cu:34-80), called
+ * from results / results_smv / results_gen (GAGA/gaga/results.cu:166-182, 336-352,
+ * 509-525).  Same keys, order and number formats ("CGIHT_S_gen output: k 20, m 256, ...
+ * seed 7 \n"); matrix_ensemble is printed for gen and smv, p for smv.  Writes the line
+ * and its NUL into buf; returns the line length, or XCS_EINVAL if buf_len is too small. */
+int xcs_results_line(char *buf, size_t buf_len, const char *alg, const char *ens, int k, int m,
+                     int n, int p, int matrix_ensemble, int vec_distribution, const float norms[3],
+                     const float times[3], int iterations, float conv_rate, const int support[4],
+                     unsigned int seed);
+
+/* Appends that line to <dir>/gpu_data_<ens>YYYYMMDD.txt (local date; dir NULL or "": the
+ * working directory), the file the reference opens with "a+" (results.cu:509-525). */
+int xcs_results_append(const char *dir, const char *alg, const char *ens, int k, int m, int n,
+                       int p, int matrix_ensemble, int vec_distribution, const float norms[3],
+                       const float times[3], int iterations, float conv_rate,
+                       const int support[4], unsigned int seed);
 
 /* Solve-given-problem forms ([outputVector iterations convRate] = gaga_cs(alg,
  * 'gen', y, A, k, options), gaga_gen.cu:115-134).  A is column-major m x n. */
