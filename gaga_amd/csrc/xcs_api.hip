@@ -1314,7 +1314,7 @@ int xcs_options_default(const char *alg, const char *ens, int n, xcs_options *o)
   o->seed = 0;
   o->instance = 0;
   o->vec_distribution = 1;                       // binary
-  o->matrix_ensemble = e == ENS_SMV ? 3 : 1;     // smv default 'ones' (gaga_smv.cu:241)
+  o->matrix_ensemble = e == ENS_SMV ? 3 : 1;     // smv default 'expander' (gaga_smv.cu:241)
   o->gpu_number = 0;
   o->mode = 0;
   o->team_size = 0;
