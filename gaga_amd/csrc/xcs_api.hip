@@ -278,6 +278,10 @@ static int parse_alg(const char *alg) {
   if (!strcmp(alg, "HTP")) return ALG_HTP;
   if (!strcmp(alg, "CSMPSP")) return ALG_CSMPSP;
   if (!strcmp(alg, "CGIHT")) return ALG_CGIHT;
+  // the reference's further greedy algorithms (gaga_gen.cu:65, 625-634)
+  if (!strcmp(alg, "IHT")) return ALG_IHT;
+  if (!strcmp(alg, "ThresholdSD")) return ALG_HTSD;
+  if (!strcmp(alg, "ThresholdCG")) return ALG_HTCG;
   return -1;
 }
 static int parse_ens(const char *ens) {
@@ -571,7 +575,9 @@ static int plan_set_smv(xcs_plan *pl, const std::vector<int> &col_ptr, const std
 }
 
 static int validate(int alg, int ens, int k, int m, int n) {
-  if (alg < 0) return set_err(XCS_EINVAL, "algorithm must be one of NIHT, HTP, CSMPSP, CGIHT");
+  if (alg < 0)
+    return set_err(XCS_EINVAL, "algorithm must be one of NIHT, HTP, CSMPSP, CGIHT, IHT, "
+                               "ThresholdSD, ThresholdCG");
   if (ens < 0) return set_err(XCS_EINVAL, "ens must be either dct, gen, or smv.");
   if (m <= 0 || n <= 0 || k < 0) return set_err(XCS_EINVAL, "need m > 0, n > 0, k >= 0");
   if (ens == ENS_DCT && (ilog2_exact(n) < 1 || n > (1 << 22)))
@@ -899,7 +905,9 @@ static int batch_configure(xcs_batch *B, int c, int block, int big = -1) {
 static int batch_generate(xcs_batch *B, int alg, int count, const int *ens, const int *k,
                           const int *m, const int *n, const int *p, const unsigned *inst,
                           const xcs_options *o) {
-  if (alg < 0) return set_err(XCS_EINVAL, "algorithm must be one of NIHT, HTP, CSMPSP, CGIHT");
+  if (alg < 0)
+    return set_err(XCS_EINVAL, "algorithm must be one of NIHT, HTP, CSMPSP, CGIHT, IHT, "
+                               "ThresholdSD, ThresholdCG");
   if (alg == ALG_CSMPSP) return set_err(XCS_EINVAL, CSMPSP_OFF);
   if (count <= 0) return set_err(XCS_EINVAL, "count must be > 0");
   for (int i = 0; i < count; i++) {

@@ -10,7 +10,15 @@ __host__ __device__ inline void xset(D &dst, S src) {
   dst = (D)src;
 }
 
-enum Alg { ALG_NIHT = 0, ALG_HTP = 1, ALG_CSMPSP = 2, ALG_CGIHT = 3 };
+enum Alg {
+  ALG_NIHT = 0,
+  ALG_HTP = 1,
+  ALG_CSMPSP = 2,
+  ALG_CGIHT = 3,
+  ALG_IHT = 4,   // IHT_S_*         (algorithms.cu:34-507)
+  ALG_HTSD = 5,  // HT_SD_S_*, "ThresholdSD" (algorithms.cu:509-924)
+  ALG_HTCG = 6   // HT_CG_S_*, "ThresholdCG" (algorithms.cu:1454-2011)
+};
 enum Ens { ENS_GEN = 0, ENS_SMV = 1, ENS_DCT = 2 };
 
 // Scalar state of one solve; lives in device memory between launches so a

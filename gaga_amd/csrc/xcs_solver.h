@@ -695,6 +695,17 @@ __device__ float step_cg_unrestricted(Team &t, Lds &L, const XProblem &P, XState
 
 // ------------------------------------------------------------ driver -------
 __device__ __forceinline__ bool should_continue(const XState &S, const XProblem &P, int *reason) {
+  if (P.alg == ALG_HTSD || P.alg == ALG_HTCG) {
+    // HT_SD_S_gen (algorithms.cu:846): err > tol, iter < maxiter, diff > 0.1 tol;
+    // HT_CG_S_gen (algorithms.cu:1913): squared errors, tol^2 and 10 tol^2
+    const float tcg = P.tol * P.tol;
+    const float lim = P.alg == ALG_HTCG ? tcg : P.tol;
+    const float dlim = P.alg == ALG_HTCG ? 10.0f * tcg : (float)(0.1 * (double)P.tol);
+    if (!(S.err > lim)) { *reason = 0; return false; }
+    if (!(S.iter < P.maxiter)) { *reason = 1; return false; }
+    if (!(S.diff > dlim)) { *reason = 3; return false; }
+    return true;
+  }
   if (!(S.err > P.tol)) { *reason = 0; return false; }
   if (!(S.iter < P.maxiter)) { *reason = 1; return false; }
   if (!(S.err < 100.0f * S.err_start)) { *reason = 2; return false; }
@@ -705,13 +716,15 @@ __device__ __forceinline__ bool should_continue(const XState &S, const XProblem 
 }
 
 // residNorm_evolution / cycle / stagnation bookkeeping (algorithms.cu:1398-1421).
-__device__ void stop_update(XState &S, int fail_after) {
+// IHT has no cycle test (algorithms.cu:422-480): cycle_test = false.
+__device__ void stop_update(XState &S, int fail_after, bool cycle_test = true) {
 #pragma unroll
   for (int j = 0; j < 15; j++) S.evo[j] = S.evo[j + 1];
   S.evo[15] = S.prev[14] - S.prev[15];
   S.diff = max16(S.evo);
-  for (int j = 0; j < 15; j++)
-    if ((S.prev[15] == S.prev[j]) && (S.iter > 4)) S.cycle = 0;
+  if (cycle_test)
+    for (int j = 0; j < 15; j++)
+      if ((S.prev[15] == S.prev[j]) && (S.iter > 4)) S.cycle = 0;
   if (S.iter > fail_after) {
     const float root = 1 / 15.0f;
     float rate = S.prev[15] / S.prev[0];
@@ -761,7 +774,8 @@ __device__ void solve_instance(Team &t, Lds &L, const XProblem &P, int budget) {
     S.k_bin = 0;
     S.k_bin_grad = 0;
     S.sum = 0;
-    S.alpha_bin = 0.25f;
+    // alpha = 0 (MaxBin = num_bins) for the one-shot thresholds (algorithms.cu:795, 1836)
+    S.alpha_bin = (P.alg == ALG_HTSD || P.alg == ALG_HTCG) ? 0.0f : 0.25f;
     S.max_bin = (int)((float)P.num_bins * (1.0f - S.alpha_bin));
     S.min_val = 0.f;
     S.max_change = (P.alg == ALG_NIHT || P.alg == ALG_CGIHT) ? 1.0f : 0.0f;
@@ -824,12 +838,19 @@ __device__ void solve_instance(Team &t, Lds &L, const XProblem &P, int budget) {
       if (w[0] > 0.0) S.max_value = (float)w[0];
     }
     S.slope = (float)(P.num_bins - 1) / S.max_value;
-    // FindSupportSet with maxChange 1 (NIHT/CGIHT) or max_value (HTP/CSMPSP):
-    // minVal = 0 is never greater, so this always bins.
-    find_support_bin(t, L, P, P.x, P.bins, S, S.slope, S.max_value, &S.k_bin);
-    threshold_x(t, P, S.k_bin);
-    S.supp_count = build_support(t, P, S.k_bin);
-    trace_write(t, P, 0, S.k_bin, S.err);
+    if (P.alg == ALG_IHT) {
+      // IHT_S_gen (algorithms.cu:397-402): the bins' slope and intercept come from
+      // A^T y and stay fixed; x starts at 0 and is first binned inside the loop.
+      for (int j = t.gtid(); j < n; j += t.gsize()) P.x[j] = 0.f;
+      S.supp_count = 0;
+    } else {
+      // FindSupportSet with maxChange 1 (NIHT/CGIHT/one-shot thresholds) or max_value
+      // (HTP/CSMPSP): minVal = 0 is never greater, so this always bins.
+      find_support_bin(t, L, P, P.x, P.bins, S, S.slope, S.max_value, &S.k_bin);
+      threshold_x(t, P, S.k_bin);
+      S.supp_count = build_support(t, P, S.k_bin);
+      trace_write(t, P, 0, S.k_bin, S.err);
+    }
     if (P.alg == ALG_CSMPSP) {
       // the initial projection counts as iteration 1 (algorithms.cu:3486-3490)
       const float errCG = cg_projection(t, L, P, S, false);
@@ -837,6 +858,24 @@ __device__ void solve_instance(Team &t, Lds &L, const XProblem &P, int budget) {
       S.prev[15] = S.err;
       S.iter = 1;
       trace_write(t, P, 1, S.k_bin, S.err);
+    } else if (P.alg == ALG_HTCG) {
+      // HT_CG_S_gen (algorithms.cu:1875-1898): r = y - A x, s = (A^T r)_S as the
+      // first direction and CG residual, mu = |s|^2, err = |r|^2 into the ring
+      double e = residual(t, L, P, S, P.resid, P.resid_update);
+      t.sync();
+      float g2 = 0.f;
+      opAT(t, L, P, S, P.resid, [&](int j, float g) {
+        const float th = (P.bins[j] <= S.k_bin) ? g : 0.f;
+        P.grad[j] = th;
+        P.gprev[j] = th;
+        g2 = fmaf(th, th, g2);
+      });
+      double w[2] = {g2, e};
+      const int o2[2] = {0, 0};
+      reduce(t, w, o2);
+      S.mu = (float)w[0];
+      S.err = (float)w[1];
+      S.prev[15] = S.err;
     } else if (P.alg == ALG_CGIHT) {
       // r^0 and its restriction (algorithms.cu:3762-3808)
       double e = residual(t, L, P, S, P.resid, P.resid_update);
@@ -864,7 +903,60 @@ __device__ void solve_instance(Team &t, Lds &L, const XProblem &P, int budget) {
 #endif
   while (budget > 0 && should_continue(S, P, &reason)) {
     budget--;
-    if (P.alg == ALG_NIHT || P.alg == ALG_HTP) {
+    if (P.alg == ALG_IHT) {
+      // IHT_S_gen loop (algorithms.cu:422-480), mu = 0.65 (gaga_gen.cu:654)
+      const float mu = 0.65f;
+      float gm = 0.f;
+      opAT(t, L, P, S, P.resid, [&](int j, float g) {
+        P.grad[j] = g;
+        gm = fmaxf(gm, fabsf(g));
+      });
+      {
+        double v[1] = {gm};
+        const int op[1] = {1};
+        reduce(t, v, op);
+        S.max_change = 2.0f * mu * (float)v[0];
+      }
+      for (int i = t.gtid(); i < n; i += t.gsize()) P.x[i] = fmaf(mu, P.grad[i], P.x[i]);
+      if (S.min_val <= S.max_change) {
+        zero_counters(t, P);
+        t.sync();
+        find_support_bin(t, L, P, P.x, P.bins, S, S.slope, S.max_value, &S.k_bin);
+      } else {
+        S.min_val = S.min_val - S.max_change;
+      }
+      threshold_x(t, P, S.k_bin);
+      t.sync();
+      S.supp_count = build_support(t, P, S.k_bin);
+      double e2 = residual(t, L, P, S, P.resid, P.resid_update);
+      double v[1] = {e2};
+      const int op[1] = {0};
+      reduce(t, v, op);
+      S.err = (float)sqrt(v[0]);
+      push16(S.prev, S.err);
+      stop_update(S, 749, false);
+    } else if (P.alg == ALG_HTSD) {
+      // HT_SD_S_gen loop (algorithms.cu:846-870): restricted steepest descent on the
+      // fixed support, then the threshold again
+      float xm;
+      (void)step_restricted_sd(t, L, P, S, &xm);
+      threshold_x(t, P, S.k_bin);
+      t.sync();
+      S.err = S.prev[15];
+#pragma unroll
+      for (int j = 0; j < 15; j++) S.evo[j] = S.evo[j + 1];
+      S.evo[15] = S.prev[14] - S.prev[15];
+      S.diff = max16(S.evo);
+    } else if (P.alg == ALG_HTCG) {
+      // HT_CG_S_gen loop (algorithms.cu:1913-1940): one restricted CG step, squared
+      // residuals in the ring
+      step_restricted_cg(t, L, P, S, S.prev);
+      S.err = S.prev[15];
+#pragma unroll
+      for (int j = 0; j < 15; j++) S.evo[j] = S.evo[j + 1];
+      S.evo[15] = S.prev[14] - S.prev[15];
+      S.diff = max16(S.evo);
+    } else if (P.alg == ALG_NIHT || P.alg == ALG_HTP) {
       float xm;
       S.max_change = step_restricted_sd(t, L, P, S, &xm);
       if (S.min_val <= S.max_change) {

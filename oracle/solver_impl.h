@@ -333,8 +333,10 @@ static void F(stop_update)(F(stop) *s, const REAL *prev, int iter, int fail_afte
   for (int j = 0; j < 15; j++) s->evo[j] = s->evo[j + 1];
   s->evo[15] = prev[14] - prev[15];
   s->diff = F(max_list)(s->evo, 16);
-  for (int j = 0; j < 15; j++)
-    if ((prev[15] == prev[j]) && (iter > 4)) s->cycle = 0;
+  if (fail_after >= 0)
+    for (int j = 0; j < 15; j++)
+      if ((prev[15] == prev[j]) && (iter > 4)) s->cycle = 0;
+  fail_after = fail_after < 0 ? -fail_after : fail_after;
   if (iter > fail_after) {
     REAL root = (REAL)(1 / 15.0f);
     REAL rate = prev[15] / prev[0];
@@ -470,16 +472,109 @@ int F(solve)(int alg, const orc_matrix *M, const float *y_in, int k, const orc_o
   int reason = 0;
   REAL tolCG = tol * tol;
   REAL tol2 = 10 * tolCG;
+  REAL max_value = 0, slope = 0;
+
+  if (alg == ORC_IHT) {
+    /* IHT_S_gen (algorithms.cu:347-507; mu = 0.65 from gaga_gen.cu:654): bins set once
+     * from A^T y (slope and intercept then fixed), x starts at 0, no cycle test. */
+    const REAL mu_iht = (REAL)0.65f;
+    F(AT)(M, w->resid, w->grad);
+    max_value = F(maxmag)(n, w->grad);
+    if (max_value == 0) {
+      res->iters = 0;
+      res->stop_reason = 0;
+      goto done;
+    }
+    slope = ((REAL)(num_bins - 1) / (max_value));
+    maxChange = 1;
+    while (F(should_continue)(&S, tol, iter, maxiter, &reason)) {
+      F(AT)(M, w->resid, w->grad);
+      maxChange = F(maxmag)(n, w->grad);
+      maxChange = 2 * mu_iht * maxChange;
+      F(axpy)(n, mu_iht, w->grad, w->x);
+      sum = F(find_support_set)(w->x, w->bins, w->counters, slope, max_value, maxChange,
+                                &minVal, &alpha, &MaxBin, &k_bin, n, k, num_bins, sum);
+      F(threshold)(n, w->x, w->bins, k_bin);
+      F(A)(M, w->x, w->resid_update);
+      memcpy(w->resid, w->y, sizeof(REAL) * mm);
+      F(axpy)(m, (REAL)-1, w->resid_update, w->resid);
+      S.err = F(nrm2)(m, w->resid);
+      F(push)(w->residNorm_prev, S.err);
+      F(stop_update)(&S, w->residNorm_prev, iter, -749);
+      iter++;
+      F(trace)(trace_kbin, trace_err, iter, k_bin, S.err);
+    }
+    res->iters = iter;
+    res->stop_reason = reason;
+    goto done;
+  }
+  if (alg == ORC_HTSD || alg == ORC_HTCG) {
+    /* HT_SD_S_gen (algorithms.cu:785-924) and HT_CG_S_gen (1826-2011): one hard
+     * threshold of A^T y with alpha = 0 (MaxBin = num_bins), then steepest descent or
+     * CG restricted to that support.  Stop rules: err > tol, iter < maxiter and
+     * residNorm_diff > 0.1 tol (SD); squared errors against tol^2 and 10 tol^2 (CG). */
+    alpha = 0;
+    MaxBin = num_bins;
+    F(AT)(M, w->y, w->x);
+    max_value = F(maxmag)(n, w->x);
+    if (max_value == 0) {
+      res->iters = 0;
+      res->stop_reason = 0;
+      goto done;
+    }
+    slope = ((REAL)(num_bins - 1) / (max_value));
+    sum = F(find_support_set)(w->x, w->bins, w->counters, slope, max_value, (REAL)1, &minVal,
+                              &alpha, &MaxBin, &k_bin, n, k, num_bins, sum);
+    F(threshold)(n, w->x, w->bins, k_bin);
+    F(trace)(trace_kbin, trace_err, 0, k_bin, S.err);
+    REAL lim, dlim;
+    if (alg == ORC_HTCG) {
+      F(A)(M, w->x, w->resid_update);
+      F(axpy)(m, (REAL)-1, w->resid_update, w->resid);
+      F(AT)(M, w->resid, w->grad);
+      F(threshold)(n, w->grad, w->bins, k_bin);
+      memcpy(w->grad_prev, w->grad, sizeof(REAL) * nn);
+      mu = F(dot)(n, w->grad_prev, w->grad_prev);
+      S.err = F(dot)(m, w->resid, w->resid);
+      w->residNorm_prev[15] = S.err;
+      lim = tolCG;
+      dlim = tol2;
+    } else {
+      lim = tol;
+      dlim = (REAL)(0.1 * tol);
+    }
+    reason = 0;
+    for (;;) {
+      if (!(S.err > lim)) { reason = 0; break; }
+      if (!(iter < maxiter)) { reason = 1; break; }
+      if (!(S.diff > dlim)) { reason = 3; break; }
+      if (alg == ORC_HTCG) {
+        F(restricted_cg)(w, &mu, k_bin, w->residNorm_prev);
+      } else {
+        (void)F(restricted_sd)(w, k_bin);
+        F(threshold)(n, w->x, w->bins, k_bin);
+      }
+      S.err = w->residNorm_prev[15];
+      for (int j = 0; j < 15; j++) S.evo[j] = S.evo[j + 1];
+      S.evo[15] = w->residNorm_prev[14] - w->residNorm_prev[15];
+      S.diff = F(max_list)(S.evo, 16);
+      iter++;
+      F(trace)(trace_kbin, trace_err, iter, k_bin, S.err);
+    }
+    res->iters = iter;
+    res->stop_reason = reason;
+    goto done;
+  }
 
   /* Set the linear bins (algorithms.cu:1340-1358 and equivalents). */
   F(AT)(M, w->y, w->x);
-  REAL max_value = F(maxmag)(n, w->x);
+  max_value = F(maxmag)(n, w->x);
   if (max_value == 0) { /* y == 0 (or y orthogonal to every column): x = 0 */
     res->iters = 0;
     res->stop_reason = 0;
     goto done;
   }
-  REAL slope = ((REAL)(num_bins - 1) / (max_value));
+  slope = ((REAL)(num_bins - 1) / (max_value));
   sum = F(find_support_set)(w->x, w->bins, w->counters, slope, max_value,
                             (alg == ORC_HTP || alg == ORC_CSMPSP) ? max_value : maxChange,
                             &minVal, &alpha, &MaxBin, &k_bin, n, k, num_bins, sum);

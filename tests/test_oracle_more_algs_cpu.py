@@ -1,0 +1,69 @@
+"""The oracle's IHT, ThresholdSD and ThresholdCG (algorithms.cu:347-507, 785-924,
+1826-2011) pinned without restating them: the one-shot thresholds end at the
+least-squares solution on their support (numpy.linalg.lstsq), the kept
+supports contain the k largest |A^T y| (the binning invariant), IHT's first
+iterate is mu A^T y on its kept set with the reference's mu = 0.65f
+(gaga_gen.cu:654), and IHT recovers planted signals inside the success region.
+"""
+import numpy as np
+import pytest
+
+SEED = 20260101
+MU = float(np.float32(0.65))
+
+
+def _problem(O, ens, m, n, k, inst):
+    P = O.make_problem(ens, m, n, SEED, inst)
+    x = O.sparse_vector(n, k, "binary", SEED, inst)
+    y = P.apply_A(x).astype(np.float32)
+    return P, x, y
+
+
+def _columns(P, S, n):
+    return np.stack([P.apply_A(np.eye(n)[j]) for j in S], axis=1)
+
+
+@pytest.mark.parametrize("ens,m,n,k", [("gen", 256, 1024, 20), ("smv", 256, 2048, 20),
+                                       ("dct", 256, 1024, 20), ("gen", 256, 1024, 5)])
+@pytest.mark.parametrize("alg,rtol", [("ThresholdCG", 1e-6), ("ThresholdSD", 1e-5)])
+def test_thresholds_end_at_least_squares(oracle, ens, m, n, k, alg, rtol):
+    P, x, y = _problem(oracle, ens, m, n, k, 1)
+    xo, res, _, _ = oracle.solve(alg, P, y, k, tol=1e-5, precision=64)
+    S = np.flatnonzero(xo)
+    assert res.stop_reason in (0, 3)
+    ls = np.linalg.lstsq(_columns(P, S, n), y.astype(np.float64), rcond=None)[0]
+    assert np.linalg.norm(xo[S] - ls) <= rtol * np.linalg.norm(ls)
+    # the support is the one-shot threshold of A^T y: it holds the k largest entries
+    g = P.apply_AT(y)
+    assert set(np.argsort(-np.abs(g))[:k]) <= set(S)
+
+
+def test_thresholds_recover_easy_problem(oracle):
+    """k = 5 of 1024 with m = 256: the k largest |A^T y| are the planted support."""
+    for inst in range(3):
+        P, x, y = _problem(oracle, "gen", 256, 1024, 5, inst)
+        for alg in ("ThresholdCG", "ThresholdSD"):
+            xo, _, _, _ = oracle.solve(alg, P, y, 5, tol=1e-5, precision=64)
+            assert np.max(np.abs(xo - x)) < 1e-5
+
+
+@pytest.mark.parametrize("ens,m,n", [("gen", 256, 1024), ("smv", 256, 2048), ("dct", 256, 1024)])
+def test_iht_first_iterate(oracle, ens, m, n):
+    k = 20
+    P, x, y = _problem(oracle, ens, m, n, k, 2)
+    x1, res, _, _ = oracle.solve("IHT", P, y, k, tol=1e-5, precision=64, maxiter=1)
+    assert res.iters == 1
+    S = np.flatnonzero(x1)
+    g = P.apply_AT(y)
+    np.testing.assert_allclose(x1[S], MU * g[S], rtol=1e-12, atol=0)
+    assert set(np.argsort(-np.abs(g))[:k]) <= set(S)
+
+
+@pytest.mark.parametrize("ens,m,n", [("gen", 256, 1024), ("smv", 256, 2048)])
+def test_iht_recovers(oracle, ens, m, n):
+    for inst in range(3):
+        P, x, y = _problem(oracle, ens, m, n, 20, inst)
+        for prec in (32, 64):
+            xo, res, _, _ = oracle.solve("IHT", P, y, 20, tol=1e-5, precision=prec)
+            assert res.stop_reason == 0
+            assert np.max(np.abs(xo - x)) < 1e-4

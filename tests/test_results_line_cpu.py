@@ -142,7 +142,7 @@ def test_bad_arguments(lib):
     import gaga_amd as ga
     o = _opts(ga, "gen")
     with pytest.raises(ga.GagaError):
-        ga.results_line("IHT", "gen", 20, 256, 1024, 0, o, NORMS, TIMES, 15, SUP, 0.5)
+        ga.results_line("NoSuchAlg", "gen", 20, 256, 1024, 0, o, NORMS, TIMES, 15, SUP, 0.5)
     with pytest.raises(ga.GagaError):
         ga.results_line("CGIHT", "xyz", 20, 256, 1024, 0, o, NORMS, TIMES, 15, SUP, 0.5)
 
