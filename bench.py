@@ -73,12 +73,14 @@ def measured_traffic(args):
     return None
 
 
-def cpu_baseline(sample):
-    """FP64 oracle on `sample` instances of the workload (bounded CPU work)."""
+def cpu_baseline(seconds, max_instances):
+    """FP64 oracle on instances of the workload, solved one after another until
+    `seconds` of solve time (or `max_instances`): a bounded sample of CPU work."""
     from oracle import oracle as O
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    ok, t_total = 0, 0.0
-    for inst in range(sample):
+    ok, t_total, count = 0, 0.0, 0
+    while count < max_instances and t_total < seconds:
+        inst = count
         P = O.make_problem("gen", M, N, SEED, inst)
         x = O.sparse_vector(N, K, "gaussian", SEED, inst)
         y = P.apply_A(x).astype(np.float32)
@@ -87,9 +89,10 @@ def cpu_baseline(sample):
         t_total += time.perf_counter() - t0
         err = np.max(np.abs(xo - x)) / np.max(np.abs(x))
         ok += int(err < SUCCESS)
+        count += 1
     return dict(value=ok / t_total, unit="problems/s", cores=threads, kind="port",
-                sample="%d CGIHT gen %dx%d k=%d instances, FP64 oracle, %.1f s" % (sample, M, N, K,
-                                                                                 t_total))
+                sample="%d CGIHT gen %dx%d k=%d instances (%d recovered), FP64 oracle, "
+                       "%.1f s of solve time" % (count, M, N, K, ok, t_total))
 
 
 def single_forms(args):
@@ -250,7 +253,9 @@ def main():
     ap.add_argument("--block", type=int, default=1024)
     ap.add_argument("--team", type=int, default=1)
     ap.add_argument("--plain-instances", type=int, default=8)
-    ap.add_argument("--cpu-sample", type=int, default=4)
+    ap.add_argument("--cpu-sample", type=int, default=400, help="most CPU-baseline instances")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0,
+                    help="CPU-baseline solve time to accumulate")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-mrhs", action="store_true")
@@ -306,7 +311,7 @@ def main():
     sweep = None if args.no_sweep else run_sweep()
     mrhs = None if args.no_mrhs else run_many_rhs()
     warm = None if args.no_warm else run_warm_start()
-    cpu = None if args.no_cpu else cpu_baseline(args.cpu_sample)
+    cpu = None if args.no_cpu else cpu_baseline(args.cpu_seconds, args.cpu_sample)
 
     out = {
         "metric": "recovered problems/s (rel. linf err < 1e-3), CGIHT gen 2048x8192 k=205, 1 GPU",
