@@ -282,6 +282,7 @@ static int parse_alg(const char *alg) {
   if (!strcmp(alg, "IHT")) return ALG_IHT;
   if (!strcmp(alg, "ThresholdSD")) return ALG_HTSD;
   if (!strcmp(alg, "ThresholdCG")) return ALG_HTCG;
+  if (!strcmp(alg, "FIHT")) return ALG_FIHT;
   return -1;
 }
 static int parse_ens(const char *ens) {
@@ -483,6 +484,7 @@ static int plan_alloc(xcs_plan *pl, int alg, int ens, int k, int m, int n, int p
   XCS_TRY(pl->alloc(&P.resid, ml));
   XCS_TRY(pl->alloc(&P.resid_update, ml));
   XCS_TRY(pl->alloc(&P.resid2, ml));
+  XCS_TRY(pl->alloc(&P.resid3, ml));
   XCS_TRY(pl->alloc(&P.counters, (size_t)P.num_bins + 64));
   XCS_TRY(pl->alloc(&P.supp, nn));
   XCS_TRY(pl->alloc(&P.wg_counts, (size_t)pl->team + 64));
@@ -577,7 +579,7 @@ static int plan_set_smv(xcs_plan *pl, const std::vector<int> &col_ptr, const std
 static int validate(int alg, int ens, int k, int m, int n) {
   if (alg < 0)
     return set_err(XCS_EINVAL, "algorithm must be one of NIHT, HTP, CSMPSP, CGIHT, IHT, "
-                               "ThresholdSD, ThresholdCG");
+                               "ThresholdSD, ThresholdCG, FIHT");
   if (ens < 0) return set_err(XCS_EINVAL, "ens must be either dct, gen, or smv.");
   if (m <= 0 || n <= 0 || k < 0) return set_err(XCS_EINVAL, "need m > 0, n > 0, k >= 0");
   if (ens == ENS_DCT && (ilog2_exact(n) < 1 || n > (1 << 22)))
@@ -869,7 +871,7 @@ static int batch_configure(xcs_batch *B, int c, int block, int big = -1) {
   float2 *cw = nullptr;
   XCS_TRY(B->walloc(&f_n, T * 4 * nn));
   XCS_TRY(B->walloc(&i_n, T * 3 * nn));
-  XCS_TRY(B->walloc(&f_m, T * 3 * ml));
+  XCS_TRY(B->walloc(&f_m, T * 4 * ml));
   XCS_TRY(B->walloc(&cnt, T * ((size_t)B->bins_max + 64)));
   XCS_TRY(B->walloc(&wgc, T * ((size_t)c + 64)));
   XCS_TRY(B->walloc(&part, T * MAXCH * ml));
@@ -884,9 +886,10 @@ static int batch_configure(xcs_batch *B, int c, int block, int big = -1) {
     xset(W.bins, i_n + (t * 3 + 0) * nn);
     xset(W.bins2, i_n + (t * 3 + 1) * nn);
     xset(W.supp, i_n + (t * 3 + 2) * nn);
-    xset(W.resid, f_m + (t * 3 + 0) * ml);
-    xset(W.resid_update, f_m + (t * 3 + 1) * ml);
-    xset(W.resid2, f_m + (t * 3 + 2) * ml);
+    xset(W.resid, f_m + (t * 4 + 0) * ml);
+    xset(W.resid_update, f_m + (t * 4 + 1) * ml);
+    xset(W.resid2, f_m + (t * 4 + 2) * ml);
+    xset(W.resid3, f_m + (t * 4 + 3) * ml);
     xset(W.counters, cnt + t * ((size_t)B->bins_max + 64));
     xset(W.wg_counts, wgc + t * ((size_t)c + 64));
     xset(W.partials, part + t * MAXCH * ml);
@@ -907,7 +910,7 @@ static int batch_generate(xcs_batch *B, int alg, int count, const int *ens, cons
                           const xcs_options *o) {
   if (alg < 0)
     return set_err(XCS_EINVAL, "algorithm must be one of NIHT, HTP, CSMPSP, CGIHT, IHT, "
-                               "ThresholdSD, ThresholdCG");
+                               "ThresholdSD, ThresholdCG, FIHT");
   if (alg == ALG_CSMPSP) return set_err(XCS_EINVAL, CSMPSP_OFF);
   if (count <= 0) return set_err(XCS_EINVAL, "count must be > 0");
   for (int i = 0; i < count; i++) {

@@ -17,7 +17,8 @@ enum Alg {
   ALG_CGIHT = 3,
   ALG_IHT = 4,   // IHT_S_*         (algorithms.cu:34-507)
   ALG_HTSD = 5,  // HT_SD_S_*, "ThresholdSD" (algorithms.cu:509-924)
-  ALG_HTCG = 6   // HT_CG_S_*, "ThresholdCG" (algorithms.cu:1454-2011)
+  ALG_HTCG = 6,  // HT_CG_S_*, "ThresholdCG" (algorithms.cu:1454-2011)
+  ALG_FIHT = 7   // FIHT_S_*        (algorithms.cu:5430-6262)
 };
 enum Ens { ENS_GEN = 0, ENS_SMV = 1, ENS_DCT = 2 };
 
@@ -55,6 +56,7 @@ struct XProblem {
   XG(float) resid;            // lda each, zero padded
   XG(float) resid_update;
   XG(float) resid2;
+  XG(float) resid3;           // FIHT: A of the extrapolated point
   XG(int) counters;           // >= num_bins
   XG(int) supp;               // n: ascending support list
   XG(int) wg_counts;          // team size
@@ -83,6 +85,7 @@ struct XWork {
   XG(float) resid;
   XG(float) resid_update;
   XG(float) resid2;
+  XG(float) resid3;
   XG(int) counters;
   XG(int) supp;
   XG(int) wg_counts;
@@ -101,6 +104,7 @@ __host__ __device__ inline void attach_work(XProblem &P, const XWork &W) {
   P.resid = W.resid;
   P.resid_update = W.resid_update;
   P.resid2 = W.resid2;
+  P.resid3 = W.resid3;
   P.counters = W.counters;
   P.supp = W.supp;
   P.wg_counts = W.wg_counts;

@@ -328,14 +328,15 @@ __device__ void find_support_bin(Team &t, Lds &L, const XProblem &P, XG(const fl
   *k_bin_out = kb;
 }
 
-// Ascending support list {i : bins[i] <= k_bin} into P.supp; returns its
-// length (uniform).  Requires bins visible; ends with supp visible.
-__device__ int build_support(Team &t, const XProblem &P, int k_bin) {
+// Ascending support list {i : in(i)} into P.supp; returns its length
+// (uniform).  Requires what `in` reads to be visible; ends with supp visible.
+template <class In>
+__device__ int build_support_if(Team &t, const XProblem &P, In &&in) {
   const int n = P.n;
   const int per = (n + t.size - 1) / t.size;
   const int lo = min(n, t.rank * per), hi = min(n, lo + per);
   int c = 0;
-  for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) c += (P.bins[i] <= k_bin);
+  for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) c += in(i) ? 1 : 0;
   int wg_total;
   wg_exclusive_scan(c, &wg_total);
   int offset = 0, total = wg_total;
@@ -369,7 +370,7 @@ __device__ int build_support(Team &t, const XProblem &P, int k_bin) {
   }
   for (int base = lo; base < hi; base += blockDim.x) {
     const int i = base + threadIdx.x;
-    const int f = (i < hi) && (P.bins[i] <= k_bin);
+    const int f = (i < hi) && in(i);
     int tot;
     const int pos = wg_exclusive_scan(f, &tot);
     if (f) P.supp[offset + pos] = i;
@@ -377,6 +378,11 @@ __device__ int build_support(Team &t, const XProblem &P, int k_bin) {
   }
   t.sync();
   return total;
+}
+
+// Support list {i : bins[i] <= k_bin}.
+__device__ int build_support(Team &t, const XProblem &P, int k_bin) {
+  return build_support_if(t, P, [&](int i) { return P.bins[i] <= k_bin; });
 }
 
 // ------------------------------------------------------------ helpers ------
@@ -876,6 +882,45 @@ __device__ void solve_instance(Team &t, Lds &L, const XProblem &P, int budget) {
       S.mu = (float)w[0];
       S.err = (float)w[1];
       S.prev[15] = S.err;
+    } else if (P.alg == ALG_FIHT) {
+      // FIHT_S_gen's extra first iteration (algorithms.cu:5513-5559): x_prev = x,
+      // x += A^T (y - A x), the threshold, A x_prev kept in resid2.
+      double e = residual(t, L, P, S, P.resid, P.resid_update);
+      {
+        double w[1] = {e};
+        const int o1[1] = {0};
+        reduce(t, w, o1);
+        S.err = (float)sqrt(w[0]);
+        S.prev[15] = S.err;
+      }
+      opAT(t, L, P, S, P.resid, [&](int j, float g) { P.grad[j] = g; });
+      t.sync();
+      float xm = 0.f;
+      for (int i = t.gtid(); i < n; i += t.gsize()) {
+        const float xi = P.x[i];
+        P.gprev[i] = xi;
+        const float xn = __fadd_rn(xi, P.grad[i]);
+        P.x[i] = xn;
+        xm = fmaxf(xm, fabsf(xn));
+      }
+      for (int i = t.gtid(); i < m; i += t.gsize()) P.resid2[i] = P.resid_update[i];
+      S.min_val = 0.f;
+      rebin_x(t, L, P, S, xm, false);
+      threshold_x(t, P, S.k_bin);
+      t.sync();
+      S.supp_count = build_support(t, P, S.k_bin);
+      e = residual(t, L, P, S, P.resid, P.resid_update);
+      {
+        double w[1] = {e};
+        const int o1[1] = {0};
+        reduce(t, w, o1);
+        S.err = (float)sqrt(w[0]);
+      }
+      push16(S.prev, S.err);
+#pragma unroll
+      for (int j = 0; j < 15; j++) S.evo[j] = S.evo[j + 1];
+      S.evo[15] = S.prev[14] - S.prev[15];
+      S.diff = max16(S.evo);
     } else if (P.alg == ALG_CGIHT) {
       // r^0 and its restriction (algorithms.cu:3762-3808)
       double e = residual(t, L, P, S, P.resid, P.resid_update);
@@ -956,6 +1001,112 @@ __device__ void solve_instance(Team &t, Lds &L, const XProblem &P, int budget) {
       for (int j = 0; j < 15; j++) S.evo[j] = S.evo[j + 1];
       S.evo[15] = S.prev[14] - S.prev[15];
       S.diff = max16(S.evo);
+    } else if (P.alg == ALG_FIHT) {
+      // RestrictedFIHT_S_gen (functions.cu:1668-1746).  Names: x_prev in gprev, the
+      // extrapolated point e in gprev_thres, A x_prev in resid2, A e (then A of the
+      // step direction) in resid3, A x in resid_update.
+      float tau, tmp;
+      {
+        double tn = 0.0, td = 0.0;
+        for (int i = t.gtid(); i < m; i += t.gsize()) {
+          const float a = __fsub_rn(P.resid2[i], P.resid_update[i]);
+          P.resid2[i] = a;
+          tn += (double)P.resid[i] * (double)a;
+          td += (double)a * (double)a;
+        }
+        double v[2] = {tn, td};
+        const int op[2] = {0, 0};
+        reduce(t, v, op);
+        tau = (float)v[0];
+        tmp = (float)v[1];
+      }
+      tau = (fabsf(tau) < 400.0f * fabsf(tmp)) ? tau / tmp : 0.f;
+      for (int i = t.gtid(); i < n; i += t.gsize()) {
+        const float xi = P.x[i];
+        P.gprev_thres[i] = fmaf(tau, __fsub_rn(P.gprev[i], xi), xi);
+      }
+      for (int i = t.gtid(); i < m; i += t.gsize()) {
+        const float ae = fmaf(tau, P.resid2[i], P.resid_update[i]);
+        P.resid3[i] = ae;
+        P.resid[i] = __fsub_rn(P.y[i], ae);
+      }
+      t.sync();
+      opAT(t, L, P, S, P.resid, [&](int j, float g) { P.grad[j] = g; });
+      t.sync();
+      double th2 = 0.0;
+      for (int i = t.gtid(); i < n; i += t.gsize()) {
+        P.gprev[i] = P.x[i];
+        const float th = (P.gprev_thres[i] == 0.f) ? 0.f : P.grad[i];  // threshold_two
+        P.thres[i] = th;
+        th2 += (double)th * (double)th;
+      }
+      for (int i = t.gtid(); i < m; i += t.gsize()) P.resid2[i] = P.resid_update[i];
+      t.sync();
+      S.supp_count = build_support_if(t, P, [&](int i) { return P.gprev_thres[i] != 0.f; });
+      opA(t, L, P, S, P.thres, P.resid3);
+      float mu;
+      {
+        double a2 = 0.0;
+        for (int i = t.gtid(); i < m; i += t.gsize()) a2 += (double)P.resid3[i] * (double)P.resid3[i];
+        double v[2] = {th2, a2};
+        const int op[2] = {0, 0};
+        reduce(t, v, op);
+        mu = (float)v[0];
+        const float den = (float)v[1];
+        mu = (mu < 400.0f * den) ? mu / den : 1.0f;
+      }
+      float xm = 0.f;
+      for (int i = t.gtid(); i < n; i += t.gsize()) {
+        const float xn = fmaf(mu, P.grad[i], P.gprev_thres[i]);
+        P.x[i] = xn;
+        xm = fmaxf(xm, fabsf(xn));
+      }
+      // the hard threshold (algorithms.cu:5571-5580)
+      S.min_val = 0.f;
+      rebin_x(t, L, P, S, xm, false);
+      threshold_x(t, P, S.k_bin);
+      t.sync();
+      S.supp_count = build_support(t, P, S.k_bin);
+      (void)residual(t, L, P, S, P.resid, P.resid_update);
+      t.sync();
+      // steepest descent on the restricted support (algorithms.cu:5581-5602)
+      const int kb = S.k_bin;
+      double s2 = 0.0;
+      opAT(t, L, P, S, P.resid, [&](int j, float g) {
+        P.grad[j] = g;
+        const float th = (P.bins[j] <= kb) ? g : 0.f;
+        P.thres[j] = th;
+        s2 += (double)th * (double)th;
+      });
+      t.sync();
+      opA(t, L, P, S, P.thres, P.resid3);
+      {
+        double a2 = 0.0;
+        for (int i = t.gtid(); i < m; i += t.gsize()) a2 += (double)P.resid3[i] * (double)P.resid3[i];
+        double v[2] = {s2, a2};
+        const int op[2] = {0, 0};
+        reduce(t, v, op);
+        mu = (float)v[0];
+        const float den = (float)v[1];
+        mu = (mu < 400.0f * den) ? mu / den : 1.0f;
+      }
+      for (int i = t.gtid(); i < n; i += t.gsize()) P.x[i] = fmaf(mu, P.thres[i], P.x[i]);
+      double e2 = 0.0;
+      for (int i = t.gtid(); i < m; i += t.gsize()) {
+        const float ru = fmaf(mu, P.resid3[i], P.resid_update[i]);
+        P.resid_update[i] = ru;
+        const float r = __fsub_rn(P.y[i], ru);
+        P.resid[i] = r;
+        e2 += (double)r * (double)r;
+      }
+      {
+        double v[1] = {e2};
+        const int op[1] = {0};
+        reduce(t, v, op);
+        S.err = (float)sqrt(v[0]);
+      }
+      push16(S.prev, S.err);
+      stop_update(S, 749);
     } else if (P.alg == ALG_NIHT || P.alg == ALG_HTP) {
       float xm;
       S.max_change = step_restricted_sd(t, L, P, S, &xm);

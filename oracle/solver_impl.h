@@ -648,6 +648,101 @@ int F(solve)(int alg, const orc_matrix *M, const float *y_in, int k, const orc_o
       iter++;
       F(trace)(trace_kbin, trace_err, iter, k_bin, S.err);
     }
+  } else if (alg == ORC_FIHT) {
+    /* FIHT_S_gen (algorithms.cu:5430-5706) with RestrictedFIHT_S_gen (functions.cu:
+     * 1668-1746): an extrapolated point e = x + tau (x_prev - x) with tau from the
+     * residual's component along A (x_prev - x), a gradient step from e on e's
+     * support, the hard threshold, then one steepest-descent step on the new support.
+     * A x_prev and A x are carried (resid_update = A x). */
+    REAL *vprev = (REAL *)calloc(nn, sizeof(REAL)), *extra = (REAL *)calloc(nn, sizeof(REAL));
+    REAL *Avprev = (REAL *)calloc(mm, sizeof(REAL)), *Aextra = (REAL *)calloc(mm, sizeof(REAL));
+    /* one more iteration (algorithms.cu:5513-5559) */
+    F(A)(M, w->x, w->resid_update);
+    F(axpy)(m, (REAL)-1, w->resid_update, w->resid);
+    S.err = F(nrm2)(m, w->resid);
+    w->residNorm_prev[15] = S.err;
+    F(AT)(M, w->resid, w->grad);
+    memcpy(vprev, w->x, sizeof(REAL) * nn);
+    memcpy(Avprev, w->resid_update, sizeof(REAL) * mm);
+    F(axpy)(n, (REAL)1, w->grad, w->x);
+    minVal = 0;
+    max_value = F(maxmag)(n, w->x);
+    slope = ((REAL)(num_bins - 1) / (max_value));
+    sum = F(find_support_set)(w->x, w->bins, w->counters, slope, max_value, (REAL)1, &minVal,
+                              &alpha, &MaxBin, &k_bin, n, k, num_bins, sum);
+    F(threshold)(n, w->x, w->bins, k_bin);
+    F(A)(M, w->x, w->resid_update);
+    memcpy(w->resid, w->y, sizeof(REAL) * mm);
+    F(axpy)(m, (REAL)-1, w->resid_update, w->resid);
+    S.err = F(nrm2)(m, w->resid);
+    F(push)(w->residNorm_prev, S.err);
+    for (int j = 0; j < 15; j++) S.evo[j] = S.evo[j + 1];
+    S.evo[15] = w->residNorm_prev[14] - w->residNorm_prev[15];
+    S.diff = F(max_list)(S.evo, 16);
+    while (F(should_continue)(&S, tol, iter, maxiter, &reason)) {
+      /* RestrictedFIHT_S_gen: the extrapolated point and the step from it */
+      F(axpy)(m, (REAL)-1, w->resid_update, Avprev);
+      REAL tau = F(dot)(m, w->resid, Avprev);
+      REAL tmp = F(dot)(m, Avprev, Avprev);
+      if (FABS(tau) < 400 * FABS(tmp))
+        tau = tau / tmp;
+      else
+        tau = 0;
+      memcpy(extra, w->x, sizeof(REAL) * nn);
+      F(axpy)(n, (REAL)-1, w->x, vprev);
+      F(axpy)(n, tau, vprev, extra);
+      memcpy(Aextra, w->resid_update, sizeof(REAL) * mm);
+      F(axpy)(m, tau, Avprev, Aextra);
+      memcpy(w->resid, w->y, sizeof(REAL) * mm);
+      F(axpy)(m, (REAL)-1, Aextra, w->resid);
+      F(AT)(M, w->resid, w->grad);
+      memcpy(vprev, w->x, sizeof(REAL) * nn);
+      memcpy(Avprev, w->resid_update, sizeof(REAL) * mm);
+      /* threshold_two (kernels.cu:53-61): the gradient on e's nonzeros */
+      for (int i = 0; i < n; i++) w->thres[i] = (extra[i] == 0) ? (REAL)0 : w->grad[i];
+      F(A)(M, w->thres, Aextra);
+      mu = F(dot)(n, w->thres, w->thres);
+      tmp = F(dot)(m, Aextra, Aextra);
+      if (mu < 400 * tmp)
+        mu = mu / tmp;
+      else
+        mu = 1;
+      memcpy(w->x, extra, sizeof(REAL) * nn);
+      F(axpy)(n, mu, w->grad, w->x);
+      /* the hard threshold (algorithms.cu:5571-5580) */
+      minVal = 0;
+      max_value = F(maxmag)(n, w->x);
+      slope = ((REAL)(num_bins - 1) / (max_value));
+      sum = F(find_support_set)(w->x, w->bins, w->counters, slope, max_value, (REAL)1, &minVal,
+                                &alpha, &MaxBin, &k_bin, n, k, num_bins, sum);
+      F(threshold)(n, w->x, w->bins, k_bin);
+      F(A)(M, w->x, w->resid_update);
+      memcpy(w->resid, w->y, sizeof(REAL) * mm);
+      F(axpy)(m, (REAL)-1, w->resid_update, w->resid);
+      /* steepest descent on the restricted support (algorithms.cu:5581-5602) */
+      F(AT)(M, w->resid, w->grad);
+      F(threshold_one)(n, w->grad, w->thres, w->bins, k_bin);
+      F(A)(M, w->thres, Aextra);
+      mu = F(dot)(n, w->thres, w->thres);
+      tmp = F(dot)(m, Aextra, Aextra);
+      if (mu < 400 * tmp)
+        mu = mu / tmp;
+      else
+        mu = 1;
+      F(axpy)(n, mu, w->thres, w->x);
+      F(axpy)(m, mu, Aextra, w->resid_update);
+      memcpy(w->resid, w->y, sizeof(REAL) * mm);
+      F(axpy)(m, (REAL)-1, w->resid_update, w->resid);
+      S.err = F(nrm2)(m, w->resid);
+      F(push)(w->residNorm_prev, S.err);
+      F(stop_update)(&S, w->residNorm_prev, iter, 749);
+      iter++;
+      F(trace)(trace_kbin, trace_err, iter, k_bin, S.err);
+    }
+    free(vprev);
+    free(extra);
+    free(Avprev);
+    free(Aextra);
   } else if (alg == ORC_CGIHT) {
     /* CGIHT_S_gen (algorithms.cu:3689-3942) */
     F(A)(M, w->x, w->resid_update);

@@ -1,4 +1,4 @@
-"""IHT, ThresholdSD and ThresholdCG on the GPU against the oracle: same support
+"""IHT, ThresholdSD, ThresholdCG and FIHT on the GPU against the oracle: same support
 as the FP32 oracle, the FP64 oracle's solution to 1e-5 (1e-4 on an instance
 IHT fails), iteration counts within 10 % (20 % where the stagnation rule
 stops), and the device and host-stepped forms bit for bit."""
@@ -8,7 +8,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 SEED = 20260101
-ALGS = ["IHT", "ThresholdSD", "ThresholdCG"]
+ALGS = ["IHT", "ThresholdSD", "ThresholdCG", "FIHT"]
 
 
 def _problem(O, ens, m, n, k, inst):
@@ -52,7 +52,7 @@ def test_matches_oracle(alg, ens, m, n, k, oracle):
             # residuals near 1e-9: rounding noise, so only the count's scale is checked
             slack = max(10, r32.iters // 2)
         assert abs(it - r32.iters) <= slack
-        if alg == "IHT" and ens != "dct" or k == 5:
+        if (alg in ("IHT", "FIHT") and ens != "dct") or alg == "FIHT" or k == 5:
             assert np.max(np.abs(xd - x)) / np.max(np.abs(x)) < 1e-3
 
 

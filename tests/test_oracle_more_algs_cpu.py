@@ -67,3 +67,20 @@ def test_iht_recovers(oracle, ens, m, n):
             xo, res, _, _ = oracle.solve("IHT", P, y, 20, tol=1e-5, precision=prec)
             assert res.stop_reason == 0
             assert np.max(np.abs(xo - x)) < 1e-4
+
+
+@pytest.mark.parametrize("ens,m,n", [("gen", 256, 1024), ("smv", 256, 2048), ("dct", 256, 1024)])
+def test_fiht_recovers(oracle, ens, m, n):
+    """FIHT_S_gen (algorithms.cu:5430-5706): the extrapolated iteration recovers the
+    planted signals of the parity problems, in both precisions, in a few iterations."""
+    for inst in range(3):
+        P, x, y = _problem(oracle, ens, m, n, 20, inst)
+        its = []
+        for prec in (32, 64):
+            xo, res, _, _ = oracle.solve("FIHT", P, y, 20, tol=1e-5, precision=prec)
+            assert res.stop_reason == 0
+            assert np.max(np.abs(xo - x)) < 1e-4
+            its.append(res.iters)
+        # faster than NIHT on the same problem (the point of the extrapolation)
+        _, rn, _, _ = oracle.solve("NIHT", P, y, 20, tol=1e-5, precision=64)
+        assert its[1] < rn.iters
