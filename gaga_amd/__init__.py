@@ -23,7 +23,7 @@ __all__ = ["cs", "options", "Plan", "Batch", "ManyRHS", "MLP", "gemm_tn", "mlp_f
            "GagaError", "load",
            "ALGORITHMS", "ENSEMBLES"]
 
-ALGORITHMS = ("NIHT", "HTP", "CSMPSP", "CGIHT", "IHT", "ThresholdSD", "ThresholdCG", "FIHT")
+ALGORITHMS = ("NIHT", "HTP", "CSMPSP", "CGIHT", "IHT", "ThresholdSD", "ThresholdCG", "FIHT", "ALPS")
 ENSEMBLES = ("gen", "smv", "dct")
 
 # gagaOptions names (gagaOptions.m:8-58) -> handler

@@ -283,6 +283,7 @@ static int parse_alg(const char *alg) {
   if (!strcmp(alg, "ThresholdSD")) return ALG_HTSD;
   if (!strcmp(alg, "ThresholdCG")) return ALG_HTCG;
   if (!strcmp(alg, "FIHT")) return ALG_FIHT;
+  if (!strcmp(alg, "ALPS")) return ALG_ALPS;
   return -1;
 }
 static int parse_ens(const char *ens) {
@@ -579,7 +580,7 @@ static int plan_set_smv(xcs_plan *pl, const std::vector<int> &col_ptr, const std
 static int validate(int alg, int ens, int k, int m, int n) {
   if (alg < 0)
     return set_err(XCS_EINVAL, "algorithm must be one of NIHT, HTP, CSMPSP, CGIHT, IHT, "
-                               "ThresholdSD, ThresholdCG, FIHT");
+                               "ThresholdSD, ThresholdCG, FIHT, ALPS");
   if (ens < 0) return set_err(XCS_EINVAL, "ens must be either dct, gen, or smv.");
   if (m <= 0 || n <= 0 || k < 0) return set_err(XCS_EINVAL, "need m > 0, n > 0, k >= 0");
   if (ens == ENS_DCT && (ilog2_exact(n) < 1 || n > (1 << 22)))
@@ -910,7 +911,7 @@ static int batch_generate(xcs_batch *B, int alg, int count, const int *ens, cons
                           const xcs_options *o) {
   if (alg < 0)
     return set_err(XCS_EINVAL, "algorithm must be one of NIHT, HTP, CSMPSP, CGIHT, IHT, "
-                               "ThresholdSD, ThresholdCG, FIHT");
+                               "ThresholdSD, ThresholdCG, FIHT, ALPS");
   if (alg == ALG_CSMPSP) return set_err(XCS_EINVAL, CSMPSP_OFF);
   if (count <= 0) return set_err(XCS_EINVAL, "count must be > 0");
   for (int i = 0; i < count; i++) {

@@ -18,7 +18,8 @@ enum Alg {
   ALG_IHT = 4,   // IHT_S_*         (algorithms.cu:34-507)
   ALG_HTSD = 5,  // HT_SD_S_*, "ThresholdSD" (algorithms.cu:509-924)
   ALG_HTCG = 6,  // HT_CG_S_*, "ThresholdCG" (algorithms.cu:1454-2011)
-  ALG_FIHT = 7   // FIHT_S_*        (algorithms.cu:5430-6262)
+  ALG_FIHT = 7,  // FIHT_S_*        (algorithms.cu:5430-6262)
+  ALG_ALPS = 8   // ALPS_S_*        (algorithms.cu:6263-7090)
 };
 enum Ens { ENS_GEN = 0, ENS_SMV = 1, ENS_DCT = 2 };
 

@@ -882,8 +882,8 @@ __device__ void solve_instance(Team &t, Lds &L, const XProblem &P, int budget) {
       S.mu = (float)w[0];
       S.err = (float)w[1];
       S.prev[15] = S.err;
-    } else if (P.alg == ALG_FIHT) {
-      // FIHT_S_gen's extra first iteration (algorithms.cu:5513-5559): x_prev = x,
+    } else if (P.alg == ALG_FIHT || P.alg == ALG_ALPS) {
+      // FIHT_S_gen's (and ALPS_S_gen's) extra first iteration (algorithms.cu:5513-5559): x_prev = x,
       // x += A^T (y - A x), the threshold, A x_prev kept in resid2.
       double e = residual(t, L, P, S, P.resid, P.resid_update);
       {
@@ -1001,10 +1001,11 @@ __device__ void solve_instance(Team &t, Lds &L, const XProblem &P, int budget) {
       for (int j = 0; j < 15; j++) S.evo[j] = S.evo[j + 1];
       S.evo[15] = S.prev[14] - S.prev[15];
       S.diff = max16(S.evo);
-    } else if (P.alg == ALG_FIHT) {
-      // RestrictedFIHT_S_gen (functions.cu:1668-1746).  Names: x_prev in gprev, the
-      // extrapolated point e in gprev_thres, A x_prev in resid2, A e (then A of the
-      // step direction) in resid3, A x in resid_update.
+    } else if (P.alg == ALG_FIHT || P.alg == ALG_ALPS) {
+      // RestrictedFIHT_S_gen (functions.cu:1668-1746) and RestrictedALPS_S_gen
+      // (functions.cu:1748-1851).  Names: x_prev in gprev, the extrapolated point e in
+      // gprev_thres, A x_prev in resid2, A e (then A of the step direction) in resid3,
+      // A x in resid_update.
       float tau, tmp;
       {
         double tn = 0.0, td = 0.0;
@@ -1034,15 +1035,51 @@ __device__ void solve_instance(Team &t, Lds &L, const XProblem &P, int budget) {
       opAT(t, L, P, S, P.resid, [&](int j, float g) { P.grad[j] = g; });
       t.sync();
       double th2 = 0.0;
-      for (int i = t.gtid(); i < n; i += t.gsize()) {
-        P.gprev[i] = P.x[i];
-        const float th = (P.gprev_thres[i] == 0.f) ? 0.f : P.grad[i];  // threshold_two
-        P.thres[i] = th;
-        th2 += (double)th * (double)th;
+      if (P.alg == ALG_ALPS) {
+        // threshold_three: the gradient off e's support, binned with the current
+        // alpha and MaxBin left unchanged (RestrictedALPS takes them by value), cut to
+        // its k largest; threshold_four adds the gradient on e's support back
+        float tm = 0.f;
+        for (int i = t.gtid(); i < n; i += t.gsize()) {
+          P.gprev[i] = P.x[i];
+          const float th = (P.gprev_thres[i] == 0.f) ? P.grad[i] : 0.f;
+          P.thres[i] = th;
+          tm = fmaxf(tm, fabsf(th));
+        }
+        for (int i = t.gtid(); i < m; i += t.gsize()) P.resid2[i] = P.resid_update[i];
+        zero_counters(t, P);
+        float mv;
+        {
+          double v[1] = {tm};
+          const int op[1] = {1};
+          reduce(t, v, op);  // barrier: counters zeroed, thres visible
+          mv = (float)v[0];
+        }
+        const float sa = S.alpha_bin;
+        const int smb = S.max_bin;
+        int kbi;
+        find_support_bin(t, L, P, P.thres, P.bins, S, (float)(P.num_bins - 1) / mv, mv, &kbi);
+        S.alpha_bin = sa;
+        S.max_bin = smb;
+        for (int i = t.gtid(); i < n; i += t.gsize()) {
+          float th = (P.bins[i] <= kbi) ? P.thres[i] : 0.f;
+          if (P.gprev_thres[i] != 0.f) th = P.grad[i];
+          P.thres[i] = th;
+          th2 += (double)th * (double)th;
+        }
+        t.sync();
+        S.supp_count = build_support_if(t, P, [&](int i) { return P.thres[i] != 0.f; });
+      } else {
+        for (int i = t.gtid(); i < n; i += t.gsize()) {
+          P.gprev[i] = P.x[i];
+          const float th = (P.gprev_thres[i] == 0.f) ? 0.f : P.grad[i];  // threshold_two
+          P.thres[i] = th;
+          th2 += (double)th * (double)th;
+        }
+        for (int i = t.gtid(); i < m; i += t.gsize()) P.resid2[i] = P.resid_update[i];
+        t.sync();
+        S.supp_count = build_support_if(t, P, [&](int i) { return P.gprev_thres[i] != 0.f; });
       }
-      for (int i = t.gtid(); i < m; i += t.gsize()) P.resid2[i] = P.resid_update[i];
-      t.sync();
-      S.supp_count = build_support_if(t, P, [&](int i) { return P.gprev_thres[i] != 0.f; });
       opA(t, L, P, S, P.thres, P.resid3);
       float mu;
       {

@@ -1,6 +1,6 @@
 /*
  * gaga_amd: MI355X-native greedy sparse recovery (NIHT, HTP, CSMPSP, CGIHT, and the
- * reference's IHT, ThresholdSD, ThresholdCG, FIHT over
+ * reference's IHT, ThresholdSD, ThresholdCG, FIHT, ALPS over
  * the 'gen', 'smv' and 'dct' ensembles).  C ABI; plain pointers and sizes.
  *
  * The entry points mirror the reference's MATLAB/MEX call surface

@@ -12,7 +12,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
 
-ALGS = {"NIHT": 0, "HTP": 1, "CSMPSP": 2, "CGIHT": 3, "IHT": 4, "ThresholdSD": 5, "ThresholdCG": 6, "FIHT": 7}
+ALGS = {"NIHT": 0, "HTP": 1, "CSMPSP": 2, "CGIHT": 3, "IHT": 4, "ThresholdSD": 5, "ThresholdCG": 6, "FIHT": 7, "ALPS": 8}
 ENS = {"gen": 0, "smv": 1, "dct": 2}
 DIST = {"uniform": 0, "binary": 1, "gaussian": 2}
 

@@ -648,12 +648,14 @@ int F(solve)(int alg, const orc_matrix *M, const float *y_in, int k, const orc_o
       iter++;
       F(trace)(trace_kbin, trace_err, iter, k_bin, S.err);
     }
-  } else if (alg == ORC_FIHT) {
+  } else if (alg == ORC_FIHT || alg == ORC_ALPS) {
     /* FIHT_S_gen (algorithms.cu:5430-5706) with RestrictedFIHT_S_gen (functions.cu:
      * 1668-1746): an extrapolated point e = x + tau (x_prev - x) with tau from the
      * residual's component along A (x_prev - x), a gradient step from e on e's
      * support, the hard threshold, then one steepest-descent step on the new support.
-     * A x_prev and A x are carried (resid_update = A x). */
+     * A x_prev and A x are carried (resid_update = A x).  ALPS_S_gen (algorithms.cu:
+     * 6263-6534) with RestrictedALPS_S_gen (functions.cu:1748-1851) differs only in
+     * the step's support: e's support plus the k largest gradient entries off it. */
     REAL *vprev = (REAL *)calloc(nn, sizeof(REAL)), *extra = (REAL *)calloc(nn, sizeof(REAL));
     REAL *Avprev = (REAL *)calloc(mm, sizeof(REAL)), *Aextra = (REAL *)calloc(mm, sizeof(REAL));
     /* one more iteration (algorithms.cu:5513-5559) */
@@ -698,8 +700,25 @@ int F(solve)(int alg, const orc_matrix *M, const float *y_in, int k, const orc_o
       F(AT)(M, w->resid, w->grad);
       memcpy(vprev, w->x, sizeof(REAL) * nn);
       memcpy(Avprev, w->resid_update, sizeof(REAL) * mm);
-      /* threshold_two (kernels.cu:53-61): the gradient on e's nonzeros */
-      for (int i = 0; i < n; i++) w->thres[i] = (extra[i] == 0) ? (REAL)0 : w->grad[i];
+      if (alg == ORC_ALPS) {
+        /* threshold_three (kernels.cu:43-51): the gradient off e's support, binned with
+         * copies of alpha, MaxBin and k_bin (RestrictedALPS takes them by value), cut to
+         * its k largest, then threshold_four (kernels.cu:32-40) adds it back on e's
+         * support */
+        for (int i = 0; i < n; i++) w->thres[i] = (extra[i] == 0) ? w->grad[i] : (REAL)0;
+        REAL mv = F(maxmag)(n, w->thres);
+        REAL sl = ((REAL)(num_bins - 1) / (mv));
+        REAL mnv = 0, al = alpha;
+        int mb = MaxBin, kbi = k_bin;
+        sum = F(find_support_set)(w->thres, w->bins, w->counters, sl, mv, (REAL)1, &mnv, &al, &mb,
+                                  &kbi, n, k, num_bins, sum);
+        F(threshold)(n, w->thres, w->bins, kbi);
+        for (int i = 0; i < n; i++)
+          if (extra[i] != 0) w->thres[i] = w->grad[i];
+      } else {
+        /* threshold_two (kernels.cu:53-61): the gradient on e's nonzeros */
+        for (int i = 0; i < n; i++) w->thres[i] = (extra[i] == 0) ? (REAL)0 : w->grad[i];
+      }
       F(A)(M, w->thres, Aextra);
       mu = F(dot)(n, w->thres, w->thres);
       tmp = F(dot)(m, Aextra, Aextra);

@@ -1,4 +1,4 @@
-"""IHT, ThresholdSD, ThresholdCG and FIHT on the GPU against the oracle: same support
+"""IHT, ThresholdSD, ThresholdCG, FIHT and ALPS on the GPU against the oracle: same support
 as the FP32 oracle, the FP64 oracle's solution to 1e-5 (1e-4 on an instance
 IHT fails), iteration counts within 10 % (20 % where the stagnation rule
 stops), and the device and host-stepped forms bit for bit."""
@@ -8,7 +8,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 SEED = 20260101
-ALGS = ["IHT", "ThresholdSD", "ThresholdCG", "FIHT"]
+ALGS = ["IHT", "ThresholdSD", "ThresholdCG", "FIHT", "ALPS"]
 
 
 def _problem(O, ens, m, n, k, inst):
@@ -39,6 +39,11 @@ def test_matches_oracle(alg, ens, m, n, k, oracle):
         xd, it, rate = _solve_dev(alg, ens, P, y, k, 1e-5)
         x32, r32, _, _ = oracle.solve(alg, P, y, k, tol=1e-5, precision=32)
         x64, r64, _, _ = oracle.solve(alg, P, y, k, tol=1e-5, precision=64)
+        if alg == "ALPS" and r64.stop_reason == 5:
+            # the cycle test ends ALPS on smv instance 0 at iteration 9 in FP32 and 13 in
+            # FP64, unrecovered in both: the device must end it unrecovered as well
+            assert np.max(np.abs(xd - x)) / np.max(np.abs(x)) > 1e-3
+            continue
         np.testing.assert_array_equal(xd != 0, x32 != 0)
         # where the stop is the stagnation rule (the one-shot thresholds, a failing IHT)
         # its iteration is rounding-sensitive: the FP32 and FP64 oracles differ by up to 5
@@ -52,7 +57,7 @@ def test_matches_oracle(alg, ens, m, n, k, oracle):
             # residuals near 1e-9: rounding noise, so only the count's scale is checked
             slack = max(10, r32.iters // 2)
         assert abs(it - r32.iters) <= slack
-        if (alg in ("IHT", "FIHT") and ens != "dct") or alg == "FIHT" or k == 5:
+        if (alg in ("IHT", "FIHT") and ens != "dct") or alg in ("FIHT", "ALPS") or k == 5:
             assert np.max(np.abs(xd - x)) / np.max(np.abs(x)) < 1e-3
 
 

@@ -84,3 +84,17 @@ def test_fiht_recovers(oracle, ens, m, n):
         # faster than NIHT on the same problem (the point of the extrapolation)
         _, rn, _, _ = oracle.solve("NIHT", P, y, 20, tol=1e-5, precision=64)
         assert its[1] < rn.iters
+
+
+@pytest.mark.parametrize("ens,m,n", [("gen", 256, 1024), ("dct", 256, 1024)])
+def test_alps_recovers(oracle, ens, m, n):
+    """ALPS_S_gen (algorithms.cu:6263-6534): recovers the planted signals of the parity
+    problems in both precisions, in fewer iterations than NIHT."""
+    for inst in range(3):
+        P, x, y = _problem(oracle, ens, m, n, 20, inst)
+        for prec in (32, 64):
+            xo, res, _, _ = oracle.solve("ALPS", P, y, 20, tol=1e-5, precision=prec)
+            assert res.stop_reason == 0
+            assert np.max(np.abs(xo - x)) < 1e-4
+        _, rn, _, _ = oracle.solve("NIHT", P, y, 20, tol=1e-5, precision=64)
+        assert res.iters < rn.iters
