@@ -226,12 +226,18 @@ def run_many_rhs():
 
 def run_warm_start():
     """Learned warm start on fresh dct instances (n = 2^16, m = n/4, k = 0.2 m)."""
+    import gaga_amd as ga
     from gaga_amd import learned
     if not os.path.exists(learned.WEIGHTS):
         return None
     net = learned.load_net()
-    rows = learned.evaluate(net, "dct", 16384, 65536, 3277, instances=6, cg_steps=3)
-    net.close()
+    try:
+        rows = learned.evaluate(net, "dct", 16384, 65536, 3277, instances=6, cg_steps=3)
+    except ga.GagaError as e:  # the library reports the warm-start solve switched off
+        log("warm start: %s" % e)
+        return {"switched_off": str(e)}
+    finally:
+        net.close()
     out = {"workload": "CGIHT dct m=16384 n=65536 k=3277 gaussian, 6 fresh instances"}
     for key in ("plain", "magnitude", "warm"):
         out[key] = dict(ms_mean=float(np.mean([r[key][0] for r in rows])),

@@ -23,7 +23,8 @@ __all__ = ["cs", "options", "Plan", "Batch", "ManyRHS", "MLP", "gemm_tn", "mlp_f
            "GagaError", "load",
            "ALGORITHMS", "ENSEMBLES"]
 
-ALGORITHMS = ("NIHT", "HTP", "CSMPSP", "CGIHT", "IHT", "ThresholdSD", "ThresholdCG", "FIHT", "ALPS")
+ALGORITHMS = ("NIHT", "HTP", "CSMPSP", "CGIHT", "IHT", "ThresholdSD", "ThresholdCG", "FIHT", "ALPS",
+              "CGIHTprojected")
 ENSEMBLES = ("gen", "smv", "dct")
 
 # gagaOptions names (gagaOptions.m:8-58) -> handler
@@ -33,8 +34,7 @@ _VEC_DIST = {"uniform": 0, "binary": 1, "gaussian": 2}
 # gaga_smv.cu:241) all ones unscaled (createMeasurements_smv.cu:178-198)
 _MAT_ENS = {"gen": {"gaussian": 1}, "smv": {"ones": 1, "binary": 2, "expander": 3}, "dct": {}}
 _ACCEPTED_IGNORED = {"threadsPerBlockn", "threadsPerBlockm", "threadsPerBlocknp",
-                     "threadsPerBlockBin", "kFixed", "alpha", "supportFlag", "timing",
-                     "projFracTol"}
+                     "threadsPerBlockBin", "kFixed", "alpha", "supportFlag", "timing"}
 
 
 def options(*args, **kwargs):
@@ -87,6 +87,9 @@ def _apply_options(alg, ens, n, opts):
             o.team_size = int(val)
         elif name == "blockSize":
             o.block_size = int(val)
+        elif name == "projFracTol":
+            # CGIHTprojected's switch tolerance (gaga_gen.cu:240-242)
+            o.proj_frac_tol = float(val)
         elif name == "convRateNum":
             if int(val) != 16:
                 raise ValueError("convRateNum other than 16 is not supported")

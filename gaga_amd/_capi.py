@@ -40,7 +40,7 @@ class Options(ctypes.Structure):
         ("seed", ctypes.c_uint), ("instance", ctypes.c_uint),
         ("vec_distribution", ctypes.c_int), ("matrix_ensemble", ctypes.c_int),
         ("gpu_number", ctypes.c_int), ("mode", ctypes.c_int), ("team_size", ctypes.c_int),
-        ("block_size", ctypes.c_int),
+        ("block_size", ctypes.c_int), ("proj_frac_tol", ctypes.c_float),
     ]
 
 

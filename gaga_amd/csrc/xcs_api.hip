@@ -284,6 +284,7 @@ static int parse_alg(const char *alg) {
   if (!strcmp(alg, "ThresholdCG")) return ALG_HTCG;
   if (!strcmp(alg, "FIHT")) return ALG_FIHT;
   if (!strcmp(alg, "ALPS")) return ALG_ALPS;
+  if (!strcmp(alg, "CGIHTprojected")) return ALG_CGIHTP;
   return -1;
 }
 static int parse_ens(const char *ens) {
@@ -453,6 +454,7 @@ static int plan_alloc(xcs_plan *pl, int alg, int ens, int k, int m, int n, int p
   P.maxiter = o->maxiter > 0 ? o->maxiter : ((alg == ALG_HTP || alg == ALG_CSMPSP) ? 300 : 5000);
   P.restart = o->restart;
   P.tol = o->tol_reference_default ? (float)(10 ^ (-4)) : o->tol;
+  P.proj_frac_tol = o->proj_frac_tol > 0.f ? o->proj_frac_tol : 3.0f;  // gaga_gen.cu:190
   P.log2n = ens == ENS_DCT ? ilog2_exact(n) : 0;
   pl->mode = o->mode;
   pl->block = o->block_size > 0 ? o->block_size : 256;
@@ -580,7 +582,7 @@ static int plan_set_smv(xcs_plan *pl, const std::vector<int> &col_ptr, const std
 static int validate(int alg, int ens, int k, int m, int n) {
   if (alg < 0)
     return set_err(XCS_EINVAL, "algorithm must be one of NIHT, HTP, CSMPSP, CGIHT, IHT, "
-                               "ThresholdSD, ThresholdCG, FIHT, ALPS");
+                               "ThresholdSD, ThresholdCG, FIHT, ALPS, CGIHTprojected");
   if (ens < 0) return set_err(XCS_EINVAL, "ens must be either dct, gen, or smv.");
   if (m <= 0 || n <= 0 || k < 0) return set_err(XCS_EINVAL, "need m > 0, n > 0, k >= 0");
   if (ens == ENS_DCT && (ilog2_exact(n) < 1 || n > (1 << 22)))
@@ -911,7 +913,7 @@ static int batch_generate(xcs_batch *B, int alg, int count, const int *ens, cons
                           const xcs_options *o) {
   if (alg < 0)
     return set_err(XCS_EINVAL, "algorithm must be one of NIHT, HTP, CSMPSP, CGIHT, IHT, "
-                               "ThresholdSD, ThresholdCG, FIHT, ALPS");
+                               "ThresholdSD, ThresholdCG, FIHT, ALPS, CGIHTprojected");
   if (alg == ALG_CSMPSP) return set_err(XCS_EINVAL, CSMPSP_OFF);
   if (count <= 0) return set_err(XCS_EINVAL, "count must be > 0");
   for (int i = 0; i < count; i++) {
@@ -964,6 +966,7 @@ static int batch_generate(xcs_batch *B, int alg, int count, const int *ens, cons
     maxiter_max = std::max(maxiter_max, P.maxiter);
     P.restart = o->restart;
     P.tol = o->tol_reference_default ? (float)(10 ^ (-4)) : o->tol;
+    P.proj_frac_tol = o->proj_frac_tol > 0.f ? o->proj_frac_tol : 3.0f;  // gaga_gen.cu:190
     P.log2n = ens[i] == ENS_DCT ? ilog2_exact(n[i]) : 0;
     const uint32_t key1 = inst ? inst[i] : (uint32_t)i;
     host_sparse_vector(P.n, P.k, o->vec_distribution, o->seed, key1, B->x_true[i], supp[i]);
@@ -1331,6 +1334,7 @@ int xcs_options_default(const char *alg, const char *ens, int n, xcs_options *o)
   o->mode = 0;
   o->team_size = 0;
   o->block_size = 0;
+  o->proj_frac_tol = 3.0f;  // gaga_gen.cu:190, gaga_smv.cu:256, gaga_dct.cu:180
   return XCS_OK;
 }
 

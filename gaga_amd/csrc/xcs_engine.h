@@ -19,7 +19,8 @@ enum Alg {
   ALG_HTSD = 5,  // HT_SD_S_*, "ThresholdSD" (algorithms.cu:509-924)
   ALG_HTCG = 6,  // HT_CG_S_*, "ThresholdCG" (algorithms.cu:1454-2011)
   ALG_FIHT = 7,  // FIHT_S_*        (algorithms.cu:5430-6262)
-  ALG_ALPS = 8   // ALPS_S_*        (algorithms.cu:6263-7090)
+  ALG_ALPS = 8,  // ALPS_S_*        (algorithms.cu:6263-7090)
+  ALG_CGIHTP = 9 // CGIHTprojected_S_* (algorithms.cu:4480-5417)
 };
 enum Ens { ENS_GEN = 0, ENS_SMV = 1, ENS_DCT = 2 };
 
@@ -29,7 +30,8 @@ struct XState {
   int initialized, done, stop_reason, iter;
   int k_bin, k_bin_grad, max_bin, sum, supp_count;
   int beta_to_zero, supp_change, fail, cycle, pad0;
-  float alpha_bin, min_val, max_change, max_value, slope, mu, err, err_start, diff, pad1;
+  float alpha_bin, min_val, max_change, max_value, slope, mu, err, err_start, diff;
+  float vt2, ru2, ap2;  // CGIHTprojected: |g_S|^2, |A g_S|^2, |A p|^2 of the last iteration
   float prev[16], evo[16];
   double mat_bytes;  // algorithmic bytes moved by the operator applications of this solve
 };
@@ -38,6 +40,7 @@ struct XState {
 struct XProblem {
   int alg, ens, m, n, k, lda, num_bins, maxiter, restart, p, nz, log2n;
   float tol;
+  float proj_frac_tol;  // CGIHTprojected 'projFracTol'
   XG(const float) A;  // gen: column-major, leading dimension lda (rows m..lda-1 are 0)
   XG(const int) col_ptr;  // smv CSC (n+1, nz)
   XG(const int) rows;

@@ -733,6 +733,11 @@ int xcs_mrhs_samples(xcs_mrhs *h, int per, unsigned int seed, float *X_out, floa
   return r;
 }
 
+static constexpr bool kWarmStartOff = true;
+static const char *const WARM_OFF =
+    "the learned warm-start solve is switched off in this build (a device fault in it is not "
+    "yet explained)";
+
 // CGIHT on a plan with the learned warm start: g = A^T y, features, network
 // scores, the k best-scored coordinates as the starting support (x = A^T y
 // on it, improved by cg_steps steps of least-squares CG, then CGIHT).  If that solve does not reach tol, plain CGIHT
@@ -742,6 +747,10 @@ int xcs_mrhs_samples(xcs_mrhs *h, int per, unsigned int seed, float *X_out, floa
 int xcs_plan_solve_warm(xcs_plan *pl, xcs_mlp *net, int cg_steps, float *ms, float *warm_ms,
                         int *fallback) {
   if (!pl || !ms) return set_err(XCS_EINVAL, "NULL argument");
+  // Switched off: one run of this solve (gen 512 x 2048, k = 40, team of two
+  // workgroups) ended in a GPU memory fault whose cause is not yet found; the
+  // solve launches nothing until it is.
+  if (kWarmStartOff) return set_err(XCS_EINVAL, WARM_OFF);
   if (cg_steps < 0 || cg_steps > 15) return set_err(XCS_EINVAL, "cg_steps must be in [0, 15]");
   if (pl->h.alg != ALG_CGIHT) return set_err(XCS_EINVAL, "the warm start is for CGIHT");
   HIP_TRY(hipSetDevice(pl->dev));

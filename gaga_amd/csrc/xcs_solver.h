@@ -937,6 +937,36 @@ __device__ void solve_instance(Team &t, Lds &L, const XProblem &P, int budget) {
       S.evo[15] = S.prev[14] - S.prev[15];
       S.beta_to_zero = 1;
       S.supp_change = 1;
+    } else if (P.alg == ALG_CGIHTP) {
+      // CGIHTprojected_S_gen initialisation (algorithms.cu:4560-4590): r = y - A x,
+      // g = A^T r, g_S = g on the support (thres), A g_S (resid_update), with the
+      // norms |g_S|^2 and |A g_S|^2 kept for the first step length
+      double e = residual(t, L, P, S, P.resid, P.resid_update);
+      t.sync();
+      double g2 = 0.0;
+      const int kb = S.k_bin;
+      opAT(t, L, P, S, P.resid, [&](int j, float g) {
+        P.grad[j] = g;
+        const float th = (P.bins[j] <= kb) ? g : 0.f;
+        P.thres[j] = th;
+        g2 += (double)th * (double)th;
+      });
+      t.sync();
+      opA(t, L, P, S, P.thres, P.resid_update);
+      double a2 = 0.0;
+      for (int i = t.gtid(); i < m; i += t.gsize()) a2 += (double)P.resid_update[i] * (double)P.resid_update[i];
+      double w[3] = {e, g2, a2};
+      const int o3[3] = {0, 0, 0};
+      reduce(t, w, o3);
+      S.err = (float)sqrt(w[0]);
+      S.prev[15] = S.err;
+      S.vt2 = (float)w[1];
+      S.ru2 = (float)w[2];
+      S.ap2 = 0.f;
+#pragma unroll
+      for (int j = 0; j < 15; j++) S.evo[j] = S.evo[j + 1];
+      S.evo[15] = S.prev[14] - S.prev[15];
+      S.supp_change = 1;
     }
     S.initialized = 1;
     t.sync();
@@ -1142,6 +1172,110 @@ __device__ void solve_instance(Team &t, Lds &L, const XProblem &P, int budget) {
         reduce(t, v, op);
         S.err = (float)sqrt(v[0]);
       }
+      push16(S.prev, S.err);
+      stop_update(S, 749);
+    } else if (P.alg == ALG_CGIHTP) {
+      // CGIHTprojected_S_gen loop (algorithms.cu:4602-4785).  Names: g_S in thres,
+      // the CG direction p in gprev, A p in resid2, A g_S in resid_update once the
+      // iteration ends.  A steepest-descent step with a fresh threshold while the
+      // gradient's part off the current support is large against its part on it
+      // (|g - p| / |g_S| > projFracTol), else a CG step on the current support.
+      const int kb0 = S.k_bin;
+      double e2, vt2, dd, ru2, ap2;
+      if (S.supp_change) {
+        float mu = (S.vt2 < 400.0f * S.ru2) ? S.vt2 / S.ru2 : 1.0f;
+        float xm = 0.f;
+        for (int i = t.gtid(); i < n; i += t.gsize()) {
+          const float xn = fmaf(mu, P.grad[i], P.x[i]);
+          P.x[i] = xn;
+          xm = fmaxf(xm, fabsf(xn));
+        }
+        S.min_val = 0.f;
+        S.max_change = 1.0f;
+        rebin_x(t, L, P, S, xm, false);
+        threshold_x(t, P, S.k_bin);
+        t.sync();
+        S.supp_count = build_support(t, P, S.k_bin);
+        e2 = residual(t, L, P, S, P.resid, P.resid_update);
+        t.sync();
+        const int kb = S.k_bin;
+        double g2 = 0.0, d2 = 0.0;
+        opAT(t, L, P, S, P.resid, [&](int j, float g) {
+          P.grad[j] = g;
+          const bool in = P.bins[j] <= kb;
+          const float th = in ? g : 0.f;
+          P.thres[j] = th;
+          P.gprev[j] = th;  // p = g_S
+          g2 += (double)th * (double)th;
+          if (!in) d2 += (double)g * (double)g;  // g - p is g off the support
+        });
+        t.sync();
+        opA(t, L, P, S, P.thres, P.resid_update);
+        double a2 = 0.0;
+        for (int i = t.gtid(); i < m; i += t.gsize()) {
+          const float ru = P.resid_update[i];
+          P.resid2[i] = ru;  // A p = A g_S
+          a2 += (double)ru * (double)ru;
+        }
+        double v[4] = {e2, g2, d2, a2};
+        const int op[4] = {0, 0, 0, 0};
+        reduce(t, v, op);
+        e2 = v[0];
+        vt2 = v[1];
+        dd = v[2];
+        ru2 = v[3];
+        ap2 = v[3];
+      } else {
+        float mu = (S.vt2 < 400.0f * S.ap2) ? S.vt2 / S.ap2 : 1.0f;
+        // p lies on the current support, so x stays on it
+        for (int i = t.gtid(); i < n; i += t.gsize()) P.x[i] = fmaf(mu, P.gprev[i], P.x[i]);
+        t.sync();
+        e2 = residual(t, L, P, S, P.resid, P.resid_update);
+        t.sync();
+        double g2 = 0.0;
+        opAT(t, L, P, S, P.resid, [&](int j, float g) {
+          P.grad[j] = g;
+          const float th = (P.bins[j] <= kb0) ? g : 0.f;
+          P.thres[j] = th;
+          g2 += (double)th * (double)th;
+        });
+        t.sync();
+        opA(t, L, P, S, P.thres, P.resid_update);
+        double v[2] = {e2, g2};
+        const int op2[2] = {0, 0};
+        reduce(t, v, op2);
+        e2 = v[0];
+        vt2 = v[1];
+        float beta = (float)vt2;
+        beta = (beta < 400.0f * S.vt2) ? beta / S.vt2 : 0.f;
+        double d2 = 0.0;
+        for (int i = t.gtid(); i < n; i += t.gsize()) {
+          const float pn = __fadd_rn(__fmul_rn(beta, P.gprev[i]), P.thres[i]);
+          P.gprev[i] = pn;
+          const float d = __fsub_rn(P.grad[i], pn);
+          d2 += (double)d * (double)d;
+        }
+        double r2 = 0.0, q2 = 0.0;
+        for (int i = t.gtid(); i < m; i += t.gsize()) {
+          const float ru = P.resid_update[i];
+          const float ap = __fadd_rn(__fmul_rn(beta, P.resid2[i]), ru);
+          P.resid2[i] = ap;
+          r2 += (double)ru * (double)ru;
+          q2 += (double)ap * (double)ap;
+        }
+        double w[3] = {d2, r2, q2};
+        const int op3[3] = {0, 0, 0};
+        reduce(t, w, op3);
+        dd = w[0];
+        ru2 = w[1];
+        ap2 = w[2];
+      }
+      S.vt2 = (float)vt2;
+      S.ru2 = (float)ru2;
+      S.ap2 = (float)ap2;
+      const float pf = (float)sqrt(dd), tn = (float)sqrt(vt2);
+      S.supp_change = (pf < 1000.0f * tn) ? ((pf / tn > P.proj_frac_tol) ? 1 : 0) : 1;
+      S.err = (float)sqrt(e2);
       push16(S.prev, S.err);
       stop_update(S, 749);
     } else if (P.alg == ALG_NIHT || P.alg == ALG_HTP) {

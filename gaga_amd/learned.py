@@ -109,17 +109,19 @@ def evaluate(net, ens, m, n, k, instances, seed=20260101, first_instance=1 << 26
         opts = ga.options(tol=1e-5, seed=seed, vecDistribution="gaussian",
                           instance=first_instance + i)
         pl = Plan("CGIHT", ens, k, m, n, p if ens == "smv" else 0, opts)
-        pl.solve()  # warm the plan
-        ms_plain = pl.solve()
-        r = pl.results()
-        plain = (ms_plain, int(r["iters"]), bool(r["norms"][2] < SUCCESS_TOL))
-        ms_w, ms_pre, fb = pl.solve_warm(net, cg_steps)
-        r = pl.results()
-        warm = (ms_w, int(r["iters"]), bool(r["norms"][2] < SUCCESS_TOL))
-        ms_m, _, fbm = pl.solve_warm(None, cg_steps)
-        r = pl.results()
-        mag = (ms_m, int(r["iters"]), bool(r["norms"][2] < SUCCESS_TOL))
-        rows.append(dict(plain=plain, warm=warm, magnitude=mag, fallback=fb,
-                         magnitude_fallback=fbm, warm_prep_ms=ms_pre))
-        pl.close()
+        try:
+            pl.solve()  # warm the plan
+            ms_plain = pl.solve()
+            r = pl.results()
+            plain = (ms_plain, int(r["iters"]), bool(r["norms"][2] < SUCCESS_TOL))
+            ms_w, ms_pre, fb = pl.solve_warm(net, cg_steps)
+            r = pl.results()
+            warm = (ms_w, int(r["iters"]), bool(r["norms"][2] < SUCCESS_TOL))
+            ms_m, _, fbm = pl.solve_warm(None, cg_steps)
+            r = pl.results()
+            mag = (ms_m, int(r["iters"]), bool(r["norms"][2] < SUCCESS_TOL))
+            rows.append(dict(plain=plain, warm=warm, magnitude=mag, fallback=fb,
+                             magnitude_fallback=fbm, warm_prep_ms=ms_pre))
+        finally:
+            pl.close()
     return rows

@@ -1,6 +1,6 @@
 /*
  * gaga_amd: MI355X-native greedy sparse recovery (NIHT, HTP, CSMPSP, CGIHT, and the
- * reference's IHT, ThresholdSD, ThresholdCG, FIHT, ALPS over
+ * reference's IHT, ThresholdSD, ThresholdCG, FIHT, ALPS, CGIHTprojected over
  * the 'gen', 'smv' and 'dct' ensembles).  C ABI; plain pointers and sizes.
  *
  * The entry points mirror the reference's MATLAB/MEX call surface
@@ -55,6 +55,8 @@ typedef struct xcs_options {
                                 one outer iteration per launch, host-side stop test */
   int team_size;             /* workgroups cooperating on one instance; 0 = auto */
   int block_size;            /* threads per workgroup; 0 = auto */
+  float proj_frac_tol;       /* 'projFracTol' of CGIHTprojected (default 3.0,
+                                gaga_gen.cu:190); <= 0 selects the default */
 } xcs_options;
 
 /* Fill *opts with the reference defaults for (alg, ens, n). */

@@ -65,7 +65,7 @@ int orc_find_support_set(const float *v, int *bins, int *counters, float slope,
 /* ---------------- operators ---------------- */
 enum { ORC_ENS_GEN = 0, ORC_ENS_SMV = 1, ORC_ENS_DCT = 2 };
 enum { ORC_NIHT = 0, ORC_HTP = 1, ORC_CSMPSP = 2, ORC_CGIHT = 3, ORC_IHT = 4, ORC_HTSD = 5,
-       ORC_HTCG = 6, ORC_FIHT = 7, ORC_ALPS = 8 };
+       ORC_HTCG = 6, ORC_FIHT = 7, ORC_ALPS = 8, ORC_CGIHTP = 9 };
 
 typedef struct {
   int ens, m, n;
@@ -90,6 +90,7 @@ typedef struct {
   int num_bins;
   int restart;      /* CGIHT restartFlag */
   int threads;      /* OpenMP threads for the operators (<=0: all) */
+  float proj_frac_tol; /* CGIHTprojected projFracTol (<= 0: 3.0, gaga_gen.cu:190) */
 } orc_opts;
 
 typedef struct {

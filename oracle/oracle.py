@@ -12,7 +12,8 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
 
-ALGS = {"NIHT": 0, "HTP": 1, "CSMPSP": 2, "CGIHT": 3, "IHT": 4, "ThresholdSD": 5, "ThresholdCG": 6, "FIHT": 7, "ALPS": 8}
+ALGS = {"NIHT": 0, "HTP": 1, "CSMPSP": 2, "CGIHT": 3, "IHT": 4, "ThresholdSD": 5, "ThresholdCG": 6, "FIHT": 7, "ALPS": 8,
+        "CGIHTprojected": 9}
 ENS = {"gen": 0, "smv": 1, "dct": 2}
 DIST = {"uniform": 0, "binary": 1, "gaussian": 2}
 
@@ -46,7 +47,8 @@ class Matrix(ctypes.Structure):
 
 class Opts(ctypes.Structure):
     _fields_ = [("tol", ctypes.c_float), ("maxiter", ctypes.c_int), ("num_bins", ctypes.c_int),
-                ("restart", ctypes.c_int), ("threads", ctypes.c_int)]
+                ("restart", ctypes.c_int), ("threads", ctypes.c_int),
+                ("proj_frac_tol", ctypes.c_float)]
 
 
 class Result(ctypes.Structure):
@@ -211,11 +213,12 @@ def make_problem(ens, m, n, seed, inst, p=7, smv_ens=2, lda=None):
 
 
 def solve(alg, prob, y, k, tol=1e-5, maxiter=None, num_bins=0, restart=0, precision=64,
-          trace=False):
+          trace=False, proj_frac_tol=3.0):
     """Run the oracle solver.  Returns (x, Result, trace_kbin, trace_err)."""
     if maxiter is None:
         maxiter = 300 if alg in ("HTP", "CSMPSP") else 5000
-    o = Opts(tol=tol, maxiter=maxiter, num_bins=num_bins, restart=restart, threads=0)
+    o = Opts(tol=tol, maxiter=maxiter, num_bins=num_bins, restart=restart, threads=0,
+             proj_frac_tol=proj_frac_tol)
     y = np.ascontiguousarray(y, dtype=np.float32)
     res = Result()
     tk = np.full(maxiter + 2, -1, dtype=np.int32) if trace else None

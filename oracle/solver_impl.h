@@ -762,6 +762,83 @@ int F(solve)(int alg, const orc_matrix *M, const float *y_in, int k, const orc_o
     free(extra);
     free(Avprev);
     free(Aextra);
+  } else if (alg == ORC_CGIHTP) {
+    /* CGIHTprojected_S_gen (algorithms.cu:4486-4789): a steepest-descent step with a
+     * fresh threshold (suppChange_flag 1), else a CG step on the current support; the
+     * flag is set when |grad - p| / |grad_S| exceeds projFracTol.  thres holds grad_S,
+     * grad_prev the direction p, resid_update A grad_S at each iteration's end. */
+    const REAL pft = (REAL)(o->proj_frac_tol > 0 ? o->proj_frac_tol : 3.0f);
+    REAL *Ap = (REAL *)calloc(mm, sizeof(REAL));
+    REAL tmp, beta;
+    /* initialisation (algorithms.cu:4560-4590) */
+    F(A)(M, w->x, w->resid_update);
+    F(axpy)(m, (REAL)-1, w->resid_update, w->resid);
+    S.err = F(nrm2)(m, w->resid);
+    w->residNorm_prev[15] = S.err;
+    F(AT)(M, w->resid, w->grad);
+    F(threshold_one)(n, w->grad, w->thres, w->bins, k_bin);
+    F(A)(M, w->thres, w->resid_update);
+    for (int j = 0; j < 15; j++) S.evo[j] = S.evo[j + 1];
+    S.evo[15] = w->residNorm_prev[14] - w->residNorm_prev[15];
+    int supp_change = 1;
+    while (F(should_continue)(&S, tol, iter, maxiter, &reason)) {
+      if (supp_change) { /* algorithms.cu:4613-4650 */
+        mu = F(dot)(n, w->thres, w->thres);
+        tmp = F(dot)(m, w->resid_update, w->resid_update);
+        mu = (mu < 400 * tmp) ? mu / tmp : (REAL)1;
+        F(axpy)(n, mu, w->grad, w->x);
+        minVal = 0;
+        maxChange = 1;
+        max_value = F(maxmag)(n, w->x);
+        slope = ((REAL)(num_bins - 1) / (max_value));
+        sum = F(find_support_set)(w->x, w->bins, w->counters, slope, max_value, maxChange,
+                                  &minVal, &alpha, &MaxBin, &k_bin, n, k, num_bins, sum);
+        F(threshold)(n, w->x, w->bins, k_bin);
+        F(A)(M, w->x, w->resid_update);
+        memcpy(w->resid, w->y, sizeof(REAL) * mm);
+        F(axpy)(m, (REAL)-1, w->resid_update, w->resid);
+        F(AT)(M, w->resid, w->grad);
+        F(threshold_one)(n, w->grad, w->thres, w->bins, k_bin);
+        F(A)(M, w->thres, w->resid_update);
+        memcpy(w->grad_prev, w->thres, sizeof(REAL) * nn);
+        memcpy(Ap, w->resid_update, sizeof(REAL) * mm);
+      } else { /* algorithms.cu:4651-4698 */
+        mu = F(dot)(n, w->thres, w->thres);
+        tmp = F(dot)(m, Ap, Ap);
+        mu = (mu < 400 * tmp) ? mu / tmp : (REAL)1;
+        F(axpy)(n, mu, w->grad_prev, w->x);
+        F(A)(M, w->x, w->resid_update);
+        memcpy(w->resid, w->y, sizeof(REAL) * mm);
+        F(axpy)(m, (REAL)-1, w->resid_update, w->resid);
+        memcpy(w->grad_prev_thres, w->thres, sizeof(REAL) * nn);
+        F(AT)(M, w->resid, w->grad);
+        F(threshold_one)(n, w->grad, w->thres, w->bins, k_bin);
+        F(A)(M, w->thres, w->resid_update);
+        beta = F(dot)(n, w->thres, w->thres);
+        tmp = F(dot)(n, w->grad_prev_thres, w->grad_prev_thres);
+        beta = (beta < 400 * tmp) ? beta / tmp : (REAL)0;
+        F(scal)(n, beta, w->grad_prev);
+        F(axpy)(n, (REAL)1, w->thres, w->grad_prev);
+        F(scal)(m, beta, Ap);
+        F(axpy)(m, (REAL)1, w->resid_update, Ap);
+      }
+      /* proj_frac = |grad - p| / |grad_S| (algorithms.cu:4729-4745); the difference
+       * vector is formed in grad_prev_thres, which the next CG step rewrites */
+      memcpy(w->grad_prev_thres, w->grad, sizeof(REAL) * nn);
+      F(axpy)(n, (REAL)-1, w->grad_prev, w->grad_prev_thres);
+      REAL pf = F(nrm2)(n, w->grad_prev_thres);
+      tmp = F(nrm2)(n, w->thres);
+      if (pf < 1000 * tmp)
+        supp_change = (pf / tmp > pft) ? 1 : 0;
+      else
+        supp_change = 1;
+      S.err = F(nrm2)(m, w->resid);
+      F(push)(w->residNorm_prev, S.err);
+      F(stop_update)(&S, w->residNorm_prev, iter, 749);
+      iter++;
+      F(trace)(trace_kbin, trace_err, iter, k_bin, S.err);
+    }
+    free(Ap);
   } else if (alg == ORC_CGIHT) {
     /* CGIHT_S_gen (algorithms.cu:3689-3942) */
     F(A)(M, w->x, w->resid_update);

@@ -57,6 +57,25 @@ def test_training_learns_a_rule():
     net.close()
 
 
+WARM_OFF = ("one run of this test ended in a GPU memory fault inside the warm-start solve whose "
+            "cause is not yet shown (a CPU run of the same solve, on the same instance, team "
+            "size and masks, under ASan, guard pages and TSan, is clean); the warm-start solve "
+            "is switched off and stays off the GPU until the cause is found")
+
+
+def test_warm_start_reports_switched_off():
+    """The switched-off entry point fails loudly and launches nothing."""
+    import gaga_amd as ga
+    pl = ga.Plan("CGIHT", "gen", 20, 256, 1024, 0,
+                 ga.options(tol=1e-5, seed=SEED, vecDistribution="binary", instance=1))
+    with pytest.raises(ga.GagaError, match="switched off"):
+        pl.solve_warm(None)
+    pl.solve()  # the plan still solves without the warm start
+    assert pl.results()["norms"][2] < 1e-3
+    pl.close()
+
+
+@pytest.mark.skip(reason=WARM_OFF)
 def test_warm_start_solves_and_falls_back():
     import gaga_amd as ga
     pl = ga.Plan("CGIHT", "gen", 40, 512, 2048, 0,
@@ -72,6 +91,7 @@ def test_warm_start_solves_and_falls_back():
     pl.close()
 
 
+@pytest.mark.skip(reason=WARM_OFF)
 def test_trained_network_gain():
     """The committed network (trained on the GPU by train_warm_start on 200k
     recoveries) against CGIHT without it on fresh subsampled-dct instances

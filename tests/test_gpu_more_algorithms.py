@@ -1,6 +1,6 @@
-"""IHT, ThresholdSD, ThresholdCG, FIHT and ALPS on the GPU against the oracle: same support
-as the FP32 oracle, the FP64 oracle's solution to 1e-5 (1e-4 on an instance
-IHT fails), iteration counts within 10 % (20 % where the stagnation rule
+"""IHT, ThresholdSD, ThresholdCG, FIHT, ALPS and CGIHTprojected on the GPU against the
+oracle: same support as the FP32 oracle, the FP64 oracle's solution to 1e-5 (1e-4 on
+an instance IHT fails), iteration counts within 10 % (20 % where the stagnation rule
 stops), and the device and host-stepped forms bit for bit."""
 import numpy as np
 import pytest
@@ -8,7 +8,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 SEED = 20260101
-ALGS = ["IHT", "ThresholdSD", "ThresholdCG", "FIHT", "ALPS"]
+ALGS = ["IHT", "ThresholdSD", "ThresholdCG", "FIHT", "ALPS", "CGIHTprojected"]
 
 
 def _problem(O, ens, m, n, k, inst):
@@ -18,9 +18,9 @@ def _problem(O, ens, m, n, k, inst):
     return P, x, y
 
 
-def _solve_dev(alg, ens, P, y, k, tol):
+def _solve_dev(alg, ens, P, y, k, tol, **kw):
     import gaga_amd as ga
-    opts = ga.options(tol=tol)
+    opts = ga.options(tol=tol, **kw)
     if ens == "gen":
         return ga.cs(alg, "gen", y, P.A.T[:P.m], k, opts)
     if ens == "smv":
@@ -57,7 +57,8 @@ def test_matches_oracle(alg, ens, m, n, k, oracle):
             # residuals near 1e-9: rounding noise, so only the count's scale is checked
             slack = max(10, r32.iters // 2)
         assert abs(it - r32.iters) <= slack
-        if (alg in ("IHT", "FIHT") and ens != "dct") or alg in ("FIHT", "ALPS") or k == 5:
+        if (alg in ("IHT", "FIHT") and ens != "dct") or alg in ("FIHT", "ALPS", "CGIHTprojected") \
+                or k == 5:
             assert np.max(np.abs(xd - x)) / np.max(np.abs(x)) < 1e-3
 
 
@@ -74,3 +75,46 @@ def test_plain_and_device_forms_identical(alg):
         pl.close()
     np.testing.assert_array_equal(res[0]["x"], res[1]["x"])
     assert res[0]["iters"] == res[1]["iters"]
+
+
+@pytest.mark.parametrize("pft", [1e30, 1e-30])
+@pytest.mark.parametrize("ens", ["gen", "dct"])
+def test_cgihtp_proj_frac_tol_reaches_the_device(pft, ens, oracle):
+    """projFracTol 1e30 keeps CG steps on the first support until the projection guard
+    trips; 1e-30 makes every step a steepest-descent step with a fresh threshold.  The
+    device follows the oracle in both (support, iterations, FP64 solution)."""
+    for inst in range(2):
+        print("case CGIHTprojected", ens, "projFracTol", pft, "instance", inst, flush=True)
+        P, x, y = _problem(oracle, ens, 256, 1024, 20, inst)
+        xd, it, _ = _solve_dev("CGIHTprojected", ens, P, y, 20, 1e-5, projFracTol=pft)
+        x32, r32, _, _ = oracle.solve("CGIHTprojected", P, y, 20, tol=1e-5, precision=32,
+                                      proj_frac_tol=pft)
+        x64, r64, _, _ = oracle.solve("CGIHTprojected", P, y, 20, tol=1e-5, precision=64,
+                                      proj_frac_tol=pft)
+        np.testing.assert_array_equal(xd != 0, x32 != 0)
+        assert np.linalg.norm(xd - x64) <= 1e-5 * np.linalg.norm(x64)
+        assert abs(it - r32.iters) <= 2
+
+
+def test_cgihtp_batch_bit_identical_to_single_instance():
+    import gaga_amd as ga
+    specs = [dict(ens="gen", k=20, m=256, n=1024, instance=0),
+             dict(ens="smv", k=12, m=128, n=1024, p=7, instance=1),
+             dict(ens="dct", k=20, m=256, n=1024, instance=2),
+             dict(ens="gen", k=9, m=100, n=300, instance=3)]
+    base = dict(tol=1e-5, seed=SEED, vecDistribution="binary")
+    B = ga.Batch("CGIHTprojected", specs, ga.options(**base))
+    B.solve()
+    rb = B.results()
+    for i, s in enumerate(specs):
+        extra = {"matrixEnsemble": "binary"} if s["ens"] == "smv" else {}
+        pl = ga.Plan("CGIHTprojected", s["ens"], s["k"], s["m"], s["n"], s.get("p", 0),
+                     ga.options(instance=s["instance"], teamSize=1, blockSize=256, **base, **extra))
+        pl.solve()
+        r = pl.results()
+        xb, _ = B.x(i)
+        np.testing.assert_array_equal(xb, r["x"])
+        assert rb["iters"][i] == r["iters"]
+        assert rb["norms"][i, 2] < 1e-3
+        pl.close()
+    B.close()
