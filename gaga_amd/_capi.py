@@ -15,7 +15,7 @@ EXPORTS = (
     "xcs_batch_create", "xcs_batch_bytes", "xcs_batch_apply_timed", "xcs_batch_solve",
     "xcs_batch_results", "xcs_batch_configure", "xcs_batch_tune",
     "xcs_batch_x", "xcs_batch_info", "xcs_batch_destroy", "xcs_batch_phase_ticks",
-    "xcs_find_support_set", "xcs_apply", "xcs_generate_gen", "xcs_gemm_tn",
+    "xcs_find_support_set", "xcs_apply", "xcs_generate_gen", "xcs_generate_smv", "xcs_gemm_tn",
     "xcs_mrhs_create", "xcs_mrhs_solve", "xcs_mrhs_results", "xcs_mrhs_x",
     "xcs_mrhs_gemm_timed", "xcs_mrhs_phase_ms", "xcs_mrhs_fp32_solve", "xcs_mrhs_configure",
     "xcs_mrhs_samples", "xcs_mrhs_gemm_tile", "xcs_mlp_create", "xcs_mlp_destroy", "xcs_mlp_param_count",
@@ -116,6 +116,8 @@ def load():
         L.xcs_find_support_set.argtypes = [_P, _I, _I, _I, _F, _F, _F, _P, _P, _P, _P, _P, _P]
         L.xcs_apply.argtypes = [_C, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P]
         L.xcs_generate_gen.argtypes = [_I, _I, ctypes.c_uint, ctypes.c_uint, _P]
+        L.xcs_generate_smv.argtypes = [_I, _I, _I, _I, ctypes.c_uint, ctypes.c_uint, _P, _P, _P,
+                                       _P, _P, _P]
         L.xcs_last_error.restype = _C
         L.xcs_version.restype = _C
         _lib = L
@@ -244,6 +246,20 @@ def generate_gen(m, n, seed, instance):
     A = np.zeros((n, m), np.float32)  # row j = column j of A
     _ck(load().xcs_generate_gen(m, n, seed, instance, _p(A)))
     return A
+
+
+def generate_smv(m, n, p, ens, seed, instance):
+    """The device smv generator: (rows, vals, row_ptr, cols, rvals, empty rows repaired)."""
+    nz = n * p
+    rows = np.empty(nz, np.int32)
+    vals = np.empty(nz, np.float32)
+    row_ptr = np.empty(m + 1, np.int32)
+    cols = np.empty(nz, np.int32)
+    rvals = np.empty(nz, np.float32)
+    empty = ctypes.c_int(0)
+    _ck(load().xcs_generate_smv(m, n, p, ens, seed, instance, _p(rows), _p(vals), _p(row_ptr),
+                                _p(cols), _p(rvals), ctypes.byref(empty)))
+    return rows, vals, row_ptr, cols, rvals, empty.value
 
 
 class Plan:

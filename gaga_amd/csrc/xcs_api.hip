@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <climits>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -13,6 +14,7 @@
 
 #include "../../include/gaga_amd.h"
 #include "xcs_solver.h"
+#include "xcs_smvgen.h"
 
 using namespace xcs;
 
@@ -341,38 +343,40 @@ static void host_sparse_vector(int n, int k, int dist, uint32_t seed, uint32_t i
     x[i] = v;
   }
 }
-// smv: p distinct uniform rows per column by rejection, ascending in the column.
-static int host_smv(int m, int n, int p, int ens, uint32_t seed, uint32_t inst,
-                    std::vector<int> &rows, std::vector<float> &vals) {
-  if (p > m) return set_err(XCS_EINVAL, "smv: p (%d) > m (%d)", p, m);
-  rows.resize((size_t)n * p);
-  vals.resize((size_t)n * p);
-  const float scale = 1.0f / sqrtf((float)p);
-  for (int j = 0; j < n; j++) {
-    int *r = rows.data() + (size_t)j * p;
-    int got = 0;
-    for (uint32_t t = 0; got < p; t++) {
-      uint32_t o[4];
-      host_draw(seed, inst, (uint32_t)j, t, STREAM_SMV_ROWS, o);
-      const int cand = (int)bounded_u32(o[0], (uint32_t)m);
-      bool dup = false;
-      for (int q = 0; q < got; q++) dup |= (r[q] == cand);
-      if (!dup) r[got++] = cand;
-    }
-    std::sort(r, r + p);
-    for (int t = 0; t < p; t++) {
-      float v = 1.0f;
-      if (ens == 2) {
-        uint32_t o[4];
-        host_draw(seed, inst, (uint32_t)j, (uint32_t)t, STREAM_SMV_VALS, o);
-        v = (o[0] & 1u) ? 1.0f : -1.0f;
-      }
-      if (ens == 1 || ens == 2) v = v * scale;
-      vals[(size_t)j * p + t] = v;
-    }
-  }
+// Device smv generation (the kernels above) into caller buffers: CSC (col_ptr
+// n+1, rows and vals n*p) and CSR (row_ptr m+1, cols and rvals n*p).  tmp holds
+// 4*m ints (counts and three scratch lists); flags[0] is a failure flag that
+// accumulates over calls (zeroed once by the caller), flags[1] receives the
+// number of empty rows the repair filled.
+struct SmvOut {
+  int *col_ptr, *rows, *row_ptr, *cols;
+  float *vals, *rvals;
+};
+static int smv_check(int m, int n, int p) {
+  if (p <= 0 || p > m) return set_err(XCS_EINVAL, "smv: need 0 < p <= m (p = %d, m = %d)", p, m);
+  if (p > SMV_P_MAX) return set_err(XCS_EINVAL, "smv: the generator supports p <= %d", SMV_P_MAX);
+  if ((long)n * p >= (long)INT_MAX) return set_err(XCS_EINVAL, "smv: n * p too large");
   return XCS_OK;
 }
+static int device_smv(hipStream_t st, int m, int n, int p, int ens, uint32_t seed, uint32_t inst,
+                      const SmvOut &d, int *tmp, int *flags) {
+  int *cnt = tmp, *first = tmp + m, *zl = tmp + 2 * (size_t)m, *don = tmp + 3 * (size_t)m;
+  const float scale = 1.0f / sqrtf((float)p);
+  HIP_TRY(hipMemsetAsync(cnt, 0, sizeof(int) * (size_t)m, st));
+  hipLaunchKernelGGL(smv_columns_kernel, dim3((n + 255) / 256), dim3(256), 0, st, m, n, p, ens,
+                     scale, seed, inst, d.rows, d.vals, cnt, flags);
+  hipLaunchKernelGGL(smv_repair_kernel, dim3(1), dim3(1024), 0, st, m, n, p, d.rows, d.vals, cnt,
+                     d.row_ptr, first, zl, don, flags + 1);
+  HIP_TRY(hipMemsetAsync(first, 0, sizeof(int) * (size_t)m, st));
+  hipLaunchKernelGGL(smv_fill_kernel, dim3((n + 256) / 256), dim3(256), 0, st, m, n, p, d.rows,
+                     d.vals, d.row_ptr, first, d.col_ptr, d.cols, d.rvals);
+  hipLaunchKernelGGL(smv_row_sort_kernel, dim3((m + 255) / 256), dim3(256), 0, st, m, d.row_ptr,
+                     d.cols, d.rvals);
+  HIP_TRY(hipGetLastError());
+  return XCS_OK;
+}
+static const char *const SMV_FAIL =
+    "smv generation: a column needed more than 2^20 draws (p too close to m)";
 
 // ------------------------------------------------------------------- plans --
 struct xcs_plan {
@@ -604,12 +608,10 @@ static int plan_generate(xcs_plan *pl, int alg, int ens, int k, int m, int n, in
   HIP_TRY(hipEventCreate(&e1));
   HIP_TRY(hipEventRecord(e0, pl->stream));
   const uint32_t seed = o->seed, inst = o->instance;
-  std::vector<int> csc_rows, csc_ptr;
-  std::vector<float> csc_vals;
+  int *smv_tmp = nullptr;
   int nz = 0;
   if (ens == ENS_SMV) {
-    if (p <= 0) return set_err(XCS_EINVAL, "smv: p must be > 0");
-    XCS_TRY(host_smv(m, n, p, o->matrix_ensemble, seed, inst, csc_rows, csc_vals));
+    XCS_TRY(smv_check(m, n, p));
     nz = n * p;
   }
   XCS_TRY(plan_alloc(pl, alg, ens, k, m, n, p, nz, o));
@@ -627,9 +629,22 @@ static int plan_generate(xcs_plan *pl, int alg, int ens, int k, int m, int n, in
     HIP_TRY(hipGetLastError());
     xset(P.A, A);
   } else if (ens == ENS_SMV) {
-    csc_ptr.resize((size_t)n + 1);
-    for (int j = 0; j <= n; j++) csc_ptr[j] = j * p;
-    XCS_TRY(plan_set_smv(pl, csc_ptr, csc_rows, csc_vals));
+    SmvOut d;
+    XCS_TRY(pl->alloc(&d.col_ptr, (size_t)n + 1));
+    XCS_TRY(pl->alloc(&d.rows, (size_t)nz));
+    XCS_TRY(pl->alloc(&d.vals, (size_t)nz));
+    XCS_TRY(pl->alloc(&d.row_ptr, (size_t)m + 1));
+    XCS_TRY(pl->alloc(&d.cols, (size_t)nz));
+    XCS_TRY(pl->alloc(&d.rvals, (size_t)nz));
+    XCS_TRY(pl->alloc(&smv_tmp, 4 * (size_t)m + 2));
+    XCS_TRY(device_smv(pl->stream, m, n, p, o->matrix_ensemble, seed, inst, d, smv_tmp,
+                       smv_tmp + 4 * (size_t)m));
+    xset(P.col_ptr, d.col_ptr);
+    xset(P.rows, d.rows);
+    xset(P.vals, d.vals);
+    xset(P.row_ptr, d.row_ptr);
+    xset(P.cols, d.cols);
+    xset(P.rvals, d.rvals);
   } else {
     std::vector<int> rows;
     host_subset(n, m, seed, inst, STREAM_DCT_ROWS, rows);
@@ -649,6 +664,11 @@ static int plan_generate(xcs_plan *pl, int alg, int ens, int k, int m, int n, in
   HIP_TRY(hipEventElapsedTime(&pl->gen_ms, e0, e1));
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  if (smv_tmp) {
+    int fl[2];
+    HIP_TRY(hipMemcpy(fl, smv_tmp + 4 * (size_t)m, sizeof(fl), hipMemcpyDeviceToHost));
+    if (fl[0]) return set_err(XCS_EINVAL, SMV_FAIL);
+  }
   return XCS_OK;
 }
 
@@ -919,8 +939,10 @@ static int batch_generate(xcs_batch *B, int alg, int count, const int *ens, cons
   for (int i = 0; i < count; i++) {
     XCS_TRY(validate(alg, ens[i], k[i], m[i], n[i]));
     if (k[i] > n[i]) return set_err(XCS_EINVAL, "instance %d: k > n", i);
-    if (ens[i] == ENS_SMV && (!p || p[i] <= 0 || p[i] > m[i]))
-      return set_err(XCS_EINVAL, "instance %d: smv needs 0 < p <= m", i);
+    if (ens[i] == ENS_SMV) {
+      if (!p) return set_err(XCS_EINVAL, "instance %d: smv needs p", i);
+      XCS_TRY(smv_check(m[i], n[i], p[i]));
+    }
   }
   XCS_TRY(use_device(o->gpu_number));
   HIP_TRY(hipGetDevice(&B->dev));
@@ -940,14 +962,14 @@ static int batch_generate(xcs_batch *B, int alg, int count, const int *ens, cons
   XCS_TRY(batch_configure(B, o->team_size > 0 ? o->team_size : 1,
                           o->block_size > 0 ? o->block_size : 256));
 
-  // descriptors and host-side draws (smv structure, dct rows, planted vectors)
+  // descriptors and host-side draws (dct rows, planted vectors); the smv
+  // structure is drawn on the device below
   std::vector<XProblem> hp((size_t)count);
   B->x_true.resize((size_t)count);
   B->ks.assign(k, k + count);
   B->last.resize((size_t)count);
-  std::vector<std::vector<int>> supp((size_t)count), smv_rows((size_t)count), dct_rows((size_t)count);
-  std::vector<std::vector<float>> smv_vals((size_t)count);
-  int maxiter_max = 0;
+  std::vector<std::vector<int>> supp((size_t)count), dct_rows((size_t)count);
+  int maxiter_max = 0, smv_m_max = 0;
   size_t small_bytes = 0, big_bytes = 0;
   std::vector<int> tw_n;
   for (int i = 0; i < count; i++) {
@@ -970,8 +992,7 @@ static int batch_generate(xcs_batch *B, int alg, int count, const int *ens, cons
     P.log2n = ens[i] == ENS_DCT ? ilog2_exact(n[i]) : 0;
     const uint32_t key1 = inst ? inst[i] : (uint32_t)i;
     host_sparse_vector(P.n, P.k, o->vec_distribution, o->seed, key1, B->x_true[i], supp[i]);
-    if (P.ens == ENS_SMV) XCS_TRY(host_smv(P.m, P.n, P.p, o->matrix_ensemble, o->seed, key1,
-                                          smv_rows[i], smv_vals[i]));
+    if (P.ens == ENS_SMV) smv_m_max = std::max(smv_m_max, P.m);
     if (P.ens == ENS_DCT) host_subset(P.n, P.m, o->seed, key1, STREAM_DCT_ROWS, dct_rows[i]);
     // small arena: y, x_true, planted support, smv/dct structure
     small_bytes += Arena::up(4 * (size_t)P.lda) + Arena::up(4 * (size_t)P.n) +
@@ -1012,6 +1033,9 @@ static int batch_generate(xcs_batch *B, int alg, int count, const int *ens, cons
   if (!(ptr)) return set_err(XCS_EINVAL, "internal: instance arena overflow")
   XCS_TRY(B->alloc(&B->xs, B->xs_len));
   XCS_TRY(B->alloc(&B->sts, (size_t)count));
+  int *smv_tmp = nullptr;
+  if (smv_m_max > 0) XCS_TRY(B->alloc(&smv_tmp, 4 * (size_t)smv_m_max + 2));
+  auto key_of = [&](int i) { return inst ? inst[i] : (uint32_t)i; };
   std::vector<std::pair<int, float2 *>> tw_cache;
   std::vector<size_t> xt_off((size_t)count);
   std::vector<const float *> xt_ptr((size_t)count);
@@ -1045,39 +1069,24 @@ static int batch_generate(xcs_batch *B, int alg, int count, const int *ens, cons
       HIP_TRY(hipGetLastError());
       xset(P.A, A);
     } else if (P.ens == ENS_SMV) {
-      const std::vector<int> &rows = smv_rows[i];
-      const std::vector<float> &vals = smv_vals[i];
       const int nz = P.nz;
-      std::vector<int> cp((size_t)P.n + 1), row_ptr((size_t)P.m + 1, 0), cols((size_t)nz);
-      std::vector<float> rvals((size_t)nz);
-      for (int j = 0; j <= P.n; j++) cp[j] = j * P.p;
-      for (int e = 0; e < nz; e++) row_ptr[rows[e] + 1]++;
-      for (int r = 0; r < P.m; r++) row_ptr[r + 1] += row_ptr[r];
-      std::vector<int> fill(row_ptr.begin(), row_ptr.end() - 1);
-      for (int j = 0; j < P.n; j++)
-        for (int e = cp[j]; e < cp[j + 1]; e++) {
-          const int pos = fill[rows[e]]++;
-          cols[pos] = j;
-          rvals[pos] = vals[e];
-        }
-      int *d_cp = small.take<int>((size_t)P.n + 1), *d_r = small.take<int>(nz);
-      float *d_v = small.take<float>(nz);
-      int *d_rp = small.take<int>((size_t)P.m + 1), *d_c = small.take<int>(nz);
-      float *d_rv = small.take<float>(nz);
-      XCS_ARENA_OK(d_cp && d_r && d_v && d_rp && d_c && d_rv);
-      HIP_TRY(hipMemcpyAsync(d_cp, cp.data(), 4 * cp.size(), hipMemcpyHostToDevice, B->stream));
-      HIP_TRY(hipMemcpyAsync(d_r, rows.data(), 4 * (size_t)nz, hipMemcpyHostToDevice, B->stream));
-      HIP_TRY(hipMemcpyAsync(d_v, vals.data(), 4 * (size_t)nz, hipMemcpyHostToDevice, B->stream));
-      HIP_TRY(hipMemcpyAsync(d_rp, row_ptr.data(), 4 * row_ptr.size(), hipMemcpyHostToDevice, B->stream));
-      HIP_TRY(hipMemcpyAsync(d_c, cols.data(), 4 * (size_t)nz, hipMemcpyHostToDevice, B->stream));
-      HIP_TRY(hipMemcpyAsync(d_rv, rvals.data(), 4 * (size_t)nz, hipMemcpyHostToDevice, B->stream));
-      HIP_TRY(hipStreamSynchronize(B->stream));  // the host vectors above go out of scope
-      xset(P.col_ptr, d_cp);
-      xset(P.rows, d_r);
-      xset(P.vals, d_v);
-      xset(P.row_ptr, d_rp);
-      xset(P.cols, d_c);
-      xset(P.rvals, d_rv);
+      SmvOut d;
+      d.col_ptr = small.take<int>((size_t)P.n + 1);
+      d.rows = small.take<int>(nz);
+      d.vals = small.take<float>(nz);
+      d.row_ptr = small.take<int>((size_t)P.m + 1);
+      d.cols = small.take<int>(nz);
+      d.rvals = small.take<float>(nz);
+      XCS_ARENA_OK(d.col_ptr && d.rows && d.vals && d.row_ptr && d.cols && d.rvals);
+      // one scratch for every instance: the launches run in order on B->stream
+      XCS_TRY(device_smv(B->stream, P.m, P.n, P.p, o->matrix_ensemble, o->seed, key_of(i), d,
+                         smv_tmp, smv_tmp + 4 * (size_t)smv_m_max));
+      xset(P.col_ptr, d.col_ptr);
+      xset(P.rows, d.rows);
+      xset(P.vals, d.vals);
+      xset(P.row_ptr, d.row_ptr);
+      xset(P.cols, d.cols);
+      xset(P.rvals, d.rvals);
     } else {
       int *d_rows = small.take<int>(P.m);
       XCS_ARENA_OK(d_rows);
@@ -1132,6 +1141,11 @@ static int batch_generate(xcs_batch *B, int alg, int count, const int *ens, cons
   HIP_TRY(hipEventElapsedTime(&B->gen_ms, e0, e1));
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  if (smv_tmp) {
+    int fl = 0;
+    HIP_TRY(hipMemcpy(&fl, smv_tmp + 4 * (size_t)smv_m_max, sizeof(int), hipMemcpyDeviceToHost));
+    if (fl) return set_err(XCS_EINVAL, SMV_FAIL);
+  }
   return XCS_OK;
 }
 
@@ -1753,6 +1767,37 @@ int xcs_generate_gen(int m, int n, unsigned int seed, unsigned int instance, flo
   return XCS_OK;
 }
 
+
+int xcs_generate_smv(int m, int n, int p, int ens, unsigned int seed, unsigned int instance,
+                     int *rows, float *vals, int *row_ptr, int *cols, float *rvals, int *empty) {
+  if (!rows || !vals) return set_err(XCS_EINVAL, "NULL argument");
+  XCS_TRY(smv_check(m, n, p));
+  XCS_TRY(use_device(0));
+  xcs_plan pl;
+  HIP_TRY(hipStreamCreateWithFlags(&pl.stream, hipStreamNonBlocking));
+  const size_t nz = (size_t)n * p;
+  SmvOut d;
+  int *tmp;
+  XCS_TRY(pl.alloc(&d.col_ptr, (size_t)n + 1));
+  XCS_TRY(pl.alloc(&d.rows, nz));
+  XCS_TRY(pl.alloc(&d.vals, nz));
+  XCS_TRY(pl.alloc(&d.row_ptr, (size_t)m + 1));
+  XCS_TRY(pl.alloc(&d.cols, nz));
+  XCS_TRY(pl.alloc(&d.rvals, nz));
+  XCS_TRY(pl.alloc(&tmp, 4 * (size_t)m + 2));
+  XCS_TRY(device_smv(pl.stream, m, n, p, ens, seed, instance, d, tmp, tmp + 4 * (size_t)m));
+  HIP_TRY(hipStreamSynchronize(pl.stream));
+  int fl[2];
+  HIP_TRY(hipMemcpy(fl, tmp + 4 * (size_t)m, sizeof(fl), hipMemcpyDeviceToHost));
+  if (fl[0]) return set_err(XCS_EINVAL, SMV_FAIL);
+  HIP_TRY(hipMemcpy(rows, d.rows, sizeof(int) * nz, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(vals, d.vals, sizeof(float) * nz, hipMemcpyDeviceToHost));
+  if (row_ptr) HIP_TRY(hipMemcpy(row_ptr, d.row_ptr, sizeof(int) * ((size_t)m + 1), hipMemcpyDeviceToHost));
+  if (cols) HIP_TRY(hipMemcpy(cols, d.cols, sizeof(int) * nz, hipMemcpyDeviceToHost));
+  if (rvals) HIP_TRY(hipMemcpy(rvals, d.rvals, sizeof(float) * nz, hipMemcpyDeviceToHost));
+  if (empty) *empty = fl[1];
+  return XCS_OK;
+}
 
 int xcs_batch_create(const char *alg, int count, const int *ens, const int *k, const int *m,
                      const int *n, const int *p, const unsigned int *instance,

@@ -270,6 +270,13 @@ int xcs_stream_copy(long bytes, int reps, float *ms_per);
 int xcs_gemm_tn(const char *fmt, int M, int N, int K, const float *A, const float *B, float *C,
                 float *A_q, float *B_q);
 int xcs_generate_gen(int m, int n, unsigned int seed, unsigned int instance, float *A);
+/* The smv generator alone (createProblem_smv.cu:31-167, createMeasurements_smv.cu:129-427,
+ * zero-row repair :263-383): p distinct rows per column (p <= 64) drawn on the device,
+ * ensemble 1 ones/sqrt(p), 2 random signs/sqrt(p), 3 ones.  rows, vals: CSC with p per
+ * column (n*p); row_ptr (m+1), cols, rvals: the CSR copy, columns ascending in a row (each
+ * may be NULL); *empty (may be NULL): empty rows the repair filled. */
+int xcs_generate_smv(int m, int n, int p, int ens, unsigned int seed, unsigned int instance,
+                     int *rows, float *vals, int *row_ptr, int *cols, float *rvals, int *empty);
 
 const char *xcs_last_error(void);
 const char *xcs_version(void);

@@ -117,7 +117,7 @@ void orc_sparse_vector(int n, int k, int dist, uint32_t seed, uint32_t inst, flo
 }
 
 int orc_smv_matrix(int m, int n, int p, int ens, uint32_t seed, uint32_t inst, int *rows,
-                   float *vals) {
+                   float *vals, int repair) {
   float scale = 1.0f / sqrtf((float)p);
   int *cnt = (int *)calloc((size_t)m, sizeof(int));
   for (int j = 0; j < n; j++) {
@@ -153,9 +153,53 @@ int orc_smv_matrix(int m, int n, int p, int ens, uint32_t seed, uint32_t inst, i
       cnt[r[t]]++;
     }
   }
+  /* Zero-row repair (createMeasurements_smv.cu:263-383): every empty row takes
+   * one entry from a row that occurs more than once.  The donors are the
+   * repeated occurrences of their row in entry order (entry j*p + t; the first
+   * occurrence of each row stays), the i-th donor goes to the i-th empty row in
+   * ascending order (the reference collects the empty rows with an atomic
+   * counter, in no fixed order), and each donor's column is sorted again with
+   * its values.  Returns the number of empty rows before the repair. */
   int empty = 0;
   for (int i = 0; i < m; i++)
     if (cnt[i] == 0) empty++;
+  if (empty > 0 && repair) {
+    int *zero = (int *)malloc(sizeof(int) * (size_t)empty);
+    int *donor = (int *)malloc(sizeof(int) * (size_t)empty);
+    unsigned char *seen = (unsigned char *)calloc((size_t)m, 1);
+    int z = 0, d = 0;
+    for (int i = 0; i < m; i++)
+      if (cnt[i] == 0) zero[z++] = i;
+    const size_t np = (size_t)n * p;
+    for (size_t e = 0; e < np && d < empty; e++) {
+      if (seen[rows[e]])
+        donor[d++] = (int)e;
+      else
+        seen[rows[e]] = 1;
+    }
+    for (int i = 0; i < d; i++) rows[donor[i]] = zero[i];
+    for (int i = 0; i < d; i++) {
+      const int j = donor[i] / p;
+      if (i > 0 && donor[i - 1] / p == j) continue;
+      int *r = rows + (size_t)j * p;
+      float *v = vals + (size_t)j * p;
+      for (int a = 1; a < p; a++) {
+        const int rv = r[a];
+        const float vv = v[a];
+        int b = a - 1;
+        while (b >= 0 && r[b] > rv) {
+          r[b + 1] = r[b];
+          v[b + 1] = v[b];
+          b--;
+        }
+        r[b + 1] = rv;
+        v[b + 1] = vv;
+      }
+    }
+    free(zero);
+    free(donor);
+    free(seen);
+  }
   free(cnt);
   return empty;
 }

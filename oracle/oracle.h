@@ -45,9 +45,10 @@ void orc_support(int n, int k, uint32_t seed, uint32_t inst, int *idx);
 /* x: zero except on orc_support(); dist 0 uniform(0,1), 1 binary +-1, 2 N(0,1). */
 void orc_sparse_vector(int n, int k, int dist, uint32_t seed, uint32_t inst, float *x);
 /* smv: p distinct rows per column (CSC with fixed p: rows[j*p+t]).
- * ens 1 ones/sqrt(p), 2 +-1/sqrt(p), 3 ones.  Returns the number of empty rows. */
+ * ens 1 ones/sqrt(p), 2 +-1/sqrt(p), 3 ones.  repair != 0: empty rows are filled
+ * by the zero-row repair.  Returns the number of empty rows before the repair. */
 int orc_smv_matrix(int m, int n, int p, int ens, uint32_t seed, uint32_t inst,
-                   int *rows, float *vals);
+                   int *rows, float *vals, int repair);
 /* dct: uniform m-subset of [0,n) row indices, ascending. */
 void orc_dct_rows(int m, int n, uint32_t seed, uint32_t inst, int *rows);
 

@@ -68,7 +68,7 @@ def _declare(L):
     L.orc_gen_matrix.argtypes = [_i, _i, _i, _u, _u, _p]
     L.orc_support.argtypes = [_i, _i, _u, _u, _p]
     L.orc_sparse_vector.argtypes = [_i, _i, _i, _u, _u, _p]
-    L.orc_smv_matrix.argtypes = [_i, _i, _i, _i, _u, _u, _p, _p]
+    L.orc_smv_matrix.argtypes = [_i, _i, _i, _i, _u, _u, _p, _p, _i]
     L.orc_smv_matrix.restype = _i
     L.orc_dct_rows.argtypes = [_i, _i, _u, _u, _p]
     L.orc_linear_binning.argtypes = [_p, _i, _i, _i, _f, _f, _p, _p]
@@ -117,10 +117,12 @@ def sparse_vector(n, k, dist, seed, inst):
     return x
 
 
-def smv_matrix(m, n, p, ens, seed, inst):
+def smv_matrix(m, n, p, ens, seed, inst, repair=True):
+    """rows, vals (CSC, p per column) and the number of empty rows before the
+    zero-row repair (applied when repair is True)."""
     rows = np.empty(n * p, dtype=np.int32)
     vals = np.empty(n * p, dtype=np.float32)
-    empty = lib().orc_smv_matrix(m, n, p, ens, seed, inst, _ptr(rows), _ptr(vals))
+    empty = lib().orc_smv_matrix(m, n, p, ens, seed, inst, _ptr(rows), _ptr(vals), int(repair))
     return rows, vals, empty
 
 

@@ -27,6 +27,28 @@ def test_gen_matrix_matches_oracle(oracle, lib):
     assert (d == 0).mean() > 0.99
 
 
+@pytest.mark.parametrize("m,n,p,ens,inst", [(1000, 1024, 2, 2, 3), (60, 64, 1, 2, 1),
+                                            (900, 1024, 2, 3, 2), (256, 2048, 7, 2, 1),
+                                            (3000, 3000, 7, 3, 9), (64, 64, 64, 2, 1),
+                                            (32768, 262144, 7, 3, 0)])
+def test_smv_generator_matches_oracle(m, n, p, ens, inst, oracle, lib):
+    """Device smv draws, zero-row repair and CSR copy equal the oracle bit for bit
+    (the first five cases have empty rows before the repair)."""
+    from gaga_amd import _capi
+    print("case smv generator", m, n, p, ens, inst, flush=True)
+    rows, vals, rp, cols, rv, empty = _capi.generate_smv(m, n, p, ens, SEED, inst)
+    r_o, v_o, e_o = oracle.smv_matrix(m, n, p, ens, SEED, inst)
+    assert empty == e_o
+    np.testing.assert_array_equal(rows, r_o)
+    np.testing.assert_array_equal(vals.view(np.int32), v_o.view(np.int32))
+    colidx = np.repeat(np.arange(n, dtype=np.int32), p)
+    order = np.lexsort((colidx, r_o))
+    np.testing.assert_array_equal(rp, np.concatenate([[0], np.cumsum(np.bincount(r_o, minlength=m))]))
+    np.testing.assert_array_equal(cols, colidx[order])
+    np.testing.assert_array_equal(rv.view(np.int32), v_o[order].view(np.int32))
+    assert np.all(np.bincount(rows, minlength=m) > 0)
+
+
 # ------------------------------------------------------------------ binning --
 def _binning_cases():
     r = _rng(1)
