@@ -34,7 +34,7 @@ _VEC_DIST = {"uniform": 0, "binary": 1, "gaussian": 2}
 # gaga_smv.cu:241) all ones unscaled (createMeasurements_smv.cu:178-198)
 _MAT_ENS = {"gen": {"gaussian": 1}, "smv": {"ones": 1, "binary": 2, "expander": 3}, "dct": {}}
 _ACCEPTED_IGNORED = {"threadsPerBlockn", "threadsPerBlockm", "threadsPerBlocknp",
-                     "threadsPerBlockBin", "kFixed", "alpha", "supportFlag", "timing"}
+                     "threadsPerBlockBin", "kFixed"}
 
 
 def options(*args, **kwargs):
@@ -87,6 +87,13 @@ def _apply_options(alg, ens, n, opts):
             o.team_size = int(val)
         elif name == "blockSize":
             o.block_size = int(val)
+        elif name == "timing":
+            # gaga_gen.cu:301-306: 'on' runs the timing variants of NIHT and HTP
+            o.timing = 1 if val in ("on", 1, True) else 0
+        elif name == "supportFlag":
+            o.supp_flag = int(val)
+        elif name == "alpha":
+            o.alpha = float(val)
         elif name == "projFracTol":
             # CGIHTprojected's switch tolerance (gaga_gen.cu:240-242)
             o.proj_frac_tol = float(val)

@@ -10,7 +10,7 @@ LIB_PATH = os.environ.get("XCS_LIB_PATH", os.path.join(_HERE, "libgaga_amd.so"))
 # every symbol include/gaga_amd.h declares
 EXPORTS = (
     "xcs_options_default", "xcs_cs_generate", "xcs_cs_gen", "xcs_cs_smv", "xcs_cs_dct",
-    "xcs_plan_create", "xcs_plan_solve", "xcs_plan_results", "xcs_plan_trace",
+    "xcs_plan_create", "xcs_plan_solve", "xcs_plan_results", "xcs_plan_trace", "xcs_plan_timings",
     "xcs_plan_destroy", "xcs_plan_bytes", "xcs_plan_apply_timed",
     "xcs_batch_create", "xcs_batch_bytes", "xcs_batch_apply_timed", "xcs_batch_solve",
     "xcs_batch_results", "xcs_batch_configure", "xcs_batch_tune",
@@ -41,6 +41,7 @@ class Options(ctypes.Structure):
         ("vec_distribution", ctypes.c_int), ("matrix_ensemble", ctypes.c_int),
         ("gpu_number", ctypes.c_int), ("mode", ctypes.c_int), ("team_size", ctypes.c_int),
         ("block_size", ctypes.c_int), ("proj_frac_tol", ctypes.c_float),
+        ("timing", ctypes.c_int), ("supp_flag", ctypes.c_int), ("alpha", ctypes.c_float),
     ]
 
 
@@ -71,6 +72,7 @@ def load():
         L.xcs_plan_solve.argtypes = [_P, _P]
         L.xcs_plan_results.argtypes = [_P, _P, _P, _P, _P, _P, _P, _P]
         L.xcs_plan_trace.argtypes = [_P, _P, _P, _I]
+        L.xcs_plan_timings.argtypes = [_P, _P, _P, _P, _P, _I]
         L.xcs_plan_destroy.argtypes = [_P]
         L.xcs_plan_destroy.restype = None
         L.xcs_plan_bytes.argtypes = [_P, _P]
@@ -320,6 +322,16 @@ class Plan:
         er = np.zeros(length, np.float32)
         _ck(load().xcs_plan_trace(self._h, _p(kb), _p(er), length))
         return kb, er
+
+    def timings(self, length):
+        """Per-iteration (iteration ms, support-set ms, CG iterations, CG ms) of the last
+        solve with timing on (NIHT, HTP)."""
+        it = np.zeros(length, np.float32)
+        sp = np.zeros(length, np.float32)
+        cg = np.zeros(length, np.int32)
+        cm = np.zeros(length, np.float32)
+        _ck(load().xcs_plan_timings(self._h, _p(it), _p(sp), _p(cg), _p(cm), length))
+        return it, sp, cg, cm
 
     def close(self):
         if self._h:

@@ -442,6 +442,20 @@ static int use_device(int gpu) {
   return XCS_OK;
 }
 
+// The timing variants exist for NIHT and HTP only (gaga_gen.cu:332-335: other
+// algorithms run their plain form).
+static int timing_fields(XProblem &P, int alg, int k, const xcs_options *o) {
+  P.timing = (o->timing && (alg == ALG_NIHT || alg == ALG_HTP)) ? 1 : 0;
+  P.supp_flag = P.timing ? o->supp_flag : 0;
+  P.alpha0 = P.timing ? o->alpha : 0.25f;
+  if (P.timing) {
+    if (o->supp_flag < 0 || o->supp_flag > 3) return set_err(XCS_EINVAL, "supportFlag must be 0, 1, 2 or 3");
+    if (!(o->alpha >= 0.f && o->alpha < 1.f)) return set_err(XCS_EINVAL, "alpha must be in [0, 1)");
+    if (o->supp_flag >= 2 && k < 1) return set_err(XCS_EINVAL, "supportFlag 2 and 3 need k >= 1");
+  }
+  return XCS_OK;
+}
+
 // Allocate workspace for (alg, ens, m, n) and fill the sizes in plan->h.
 static int plan_alloc(xcs_plan *pl, int alg, int ens, int k, int m, int n, int p, int nz,
                       const xcs_options *o) {
@@ -459,6 +473,7 @@ static int plan_alloc(xcs_plan *pl, int alg, int ens, int k, int m, int n, int p
   P.restart = o->restart;
   P.tol = o->tol_reference_default ? (float)(10 ^ (-4)) : o->tol;
   P.proj_frac_tol = o->proj_frac_tol > 0.f ? o->proj_frac_tol : 3.0f;  // gaga_gen.cu:190
+  XCS_TRY(timing_fields(P, alg, k, o));
   P.log2n = ens == ENS_DCT ? ilog2_exact(n) : 0;
   pl->mode = o->mode;
   pl->block = o->block_size > 0 ? o->block_size : 256;
@@ -508,6 +523,7 @@ static int plan_alloc(xcs_plan *pl, int alg, int ens, int k, int m, int n, int p
   XCS_TRY(pl->alloc(&P.st, 1));
   XCS_TRY(pl->alloc(&P.trace_err, (size_t)P.maxiter + 2));
   XCS_TRY(pl->alloc(&P.trace_kbin, (size_t)P.maxiter + 2));
+  if (P.timing) XCS_TRY(pl->alloc(&P.trace_ticks, 4 * ((size_t)P.maxiter + 2)));
   XCS_TRY(pl->alloc(&pl->red, (size_t)2 * RED_SLOTS * (pl->team + 1)));
   XCS_TRY(pl->alloc(&pl->bar, (size_t)BAR_WORDS));
   XCS_TRY(pl->alloc(&pl->d, 1));
@@ -989,6 +1005,7 @@ static int batch_generate(xcs_batch *B, int alg, int count, const int *ens, cons
     P.restart = o->restart;
     P.tol = o->tol_reference_default ? (float)(10 ^ (-4)) : o->tol;
     P.proj_frac_tol = o->proj_frac_tol > 0.f ? o->proj_frac_tol : 3.0f;  // gaga_gen.cu:190
+    XCS_TRY(timing_fields(P, alg, k[i], o));
     P.log2n = ens[i] == ENS_DCT ? ilog2_exact(n[i]) : 0;
     const uint32_t key1 = inst ? inst[i] : (uint32_t)i;
     host_sparse_vector(P.n, P.k, o->vec_distribution, o->seed, key1, B->x_true[i], supp[i]);
@@ -1349,6 +1366,9 @@ int xcs_options_default(const char *alg, const char *ens, int n, xcs_options *o)
   o->team_size = 0;
   o->block_size = 0;
   o->proj_frac_tol = 3.0f;  // gaga_gen.cu:190, gaga_smv.cu:256, gaga_dct.cu:180
+  o->timing = 0;            // gaga_gen.cu:185
+  o->supp_flag = 0;         // gaga_gen.cu:187
+  o->alpha = 0.25f;         // gaga_gen.cu:186
   return XCS_OK;
 }
 
@@ -1767,6 +1787,27 @@ int xcs_generate_gen(int m, int n, unsigned int seed, unsigned int instance, flo
   return XCS_OK;
 }
 
+
+int xcs_plan_timings(xcs_plan *pl, float *iter_ms, float *supp_ms, int *cg_iters, float *cg_ms,
+                     int len) {
+  if (!pl || len < 0) return set_err(XCS_EINVAL, "bad arguments");
+  if (!pl->h.trace_ticks) return set_err(XCS_EINVAL, "the plan was not created with timing on (NIHT or HTP)");
+  len = std::min(len, pl->h.maxiter + 2);
+  HIP_TRY(hipSetDevice(pl->dev));
+  std::vector<unsigned long long> t(4 * (size_t)len);
+  if (len) HIP_TRY(hipMemcpy(t.data(), pl->h.trace_ticks, sizeof(unsigned long long) * t.size(), hipMemcpyDeviceToHost));
+  int khz = 0;
+  HIP_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, pl->dev));
+  if (khz <= 0) return set_err(XCS_EHIP, "wall clock rate unavailable");
+  const double per_ms = (double)khz;
+  for (int i = 0; i < len; i++) {
+    if (iter_ms) iter_ms[i] = (float)(t[4 * i + 0] / per_ms);
+    if (supp_ms) supp_ms[i] = (float)(t[4 * i + 1] / per_ms);
+    if (cg_iters) cg_iters[i] = (int)t[4 * i + 2];
+    if (cg_ms) cg_ms[i] = (float)(t[4 * i + 3] / per_ms);
+  }
+  return XCS_OK;
+}
 
 int xcs_generate_smv(int m, int n, int p, int ens, unsigned int seed, unsigned int instance,
                      int *rows, float *vals, int *row_ptr, int *cols, float *rvals, int *empty) {

@@ -32,6 +32,7 @@ struct XState {
   int beta_to_zero, supp_change, fail, cycle, pad0;
   float alpha_bin, min_val, max_change, max_value, slope, mu, err, err_start, diff;
   float vt2, ru2, ap2;  // CGIHTprojected: |g_S|^2, |A g_S|^2, |A p|^2 of the last iteration
+  float thr;            // NIHT/HTP supportFlag 2-3: the running magnitude threshold T
   float prev[16], evo[16];
   double mat_bytes;  // algorithmic bytes moved by the operator applications of this solve
 };
@@ -73,6 +74,11 @@ struct XProblem {
   XG(unsigned long long) phase_ticks;  // optional (XCS_PHASE_TIMING builds): 32 counters
   XG(const int) warm;         // optional CGIHT warm start: support mask (n), 1 = in
   int warm_cg;                // CG steps of the least-squares start on that support
+  // NIHT/HTP timing variants (algorithms_timings.cu): supportFlag 0-3, alpha_start,
+  // and per iteration 4 clock counts (iteration, support step, CG steps, CG)
+  int timing, supp_flag;
+  float alpha0;
+  XG(unsigned long long) trace_ticks;  // optional, 4 * (maxiter + 2)
 };
 
 constexpr int MAXCH = 64;  // gen gather: max column chunks (partials rows)

@@ -513,7 +513,7 @@ __device__ void step_restricted_cg(Team &t, Lds &L, const XProblem &P, XState &S
 // CG projection on the current support shared by HTP and CSMPSP
 // (algorithms.cu:2607-2669, 3431-3488, 3548-3604).  Returns errCG.
 __device__ float cg_projection(Team &t, Lds &L, const XProblem &P, XState &S, bool fixed_diff,
-                               int max_steps = 15) {
+                               int max_steps = 15, int *iters = nullptr) {
   const int kb = S.k_bin;
   float hist[16], evo[16];
 #pragma unroll
@@ -550,6 +550,7 @@ __device__ float cg_projection(Team &t, Lds &L, const XProblem &P, XState &S, bo
     evo[15] = hist[14] - hist[15];
     diffCG = fixed_diff ? 10.0f : max16(evo);
   }
+  if (iters) *iters = iterCG;
   return errCG;
 }
 
@@ -739,6 +740,38 @@ __device__ void stop_update(XState &S, int fail_after, bool cycle_test = true) {
   }
 }
 
+// The timing variants' bookkeeping (algorithms_timings.cu:607-619, 1525-1539): no
+// cycle test, the stagnation rate with root 1/16.
+__device__ void stop_update_timing(XState &S, int fail_after) {
+#pragma unroll
+  for (int j = 0; j < 15; j++) S.evo[j] = S.evo[j + 1];
+  S.evo[15] = S.prev[14] - S.prev[15];
+  S.diff = max16(S.evo);
+  if (S.iter > fail_after) {
+    const float root = 1 / 16.0f;
+    float rate = S.prev[15] / S.prev[0];
+    rate = powf(rate, root);
+    if (rate > 0.999f) S.fail = 1;
+  }
+}
+
+// The k-th largest |v| over the team (the value a sort of the magnitudes puts
+// at n - k, functions.cu:451-452): bitwise search over the bit patterns of the
+// non-negative floats, one exact team count per bit.  k >= 1.
+__device__ float team_kth_largest_abs(Team &t, const XProblem &P, XG(const float) v, int k) {
+  uint32_t thr = 0;
+  for (int bit = 30; bit >= 0; bit--) {
+    const uint32_t cand = thr | (1u << bit);
+    int c = 0;
+    for (int i = t.gtid(); i < P.n; i += t.gsize()) c += (__float_as_uint(fabsf(v[i])) >= cand) ? 1 : 0;
+    double w[1] = {(double)c};
+    const int op[1] = {0};
+    team_reduce<double, 1>(t, w, op);
+    if ((long)w[0] >= (long)k) thr = cand;
+  }
+  return __uint_as_float(thr);
+}
+
 __device__ void trace_write(const Team &t, const XProblem &P, int idx, int kb, float e) {
   if (t.rank == 0 && threadIdx.x == 0) {
     if (P.trace_kbin) P.trace_kbin[idx] = kb;
@@ -781,7 +814,8 @@ __device__ void solve_instance(Team &t, Lds &L, const XProblem &P, int budget) {
     S.k_bin_grad = 0;
     S.sum = 0;
     // alpha = 0 (MaxBin = num_bins) for the one-shot thresholds (algorithms.cu:795, 1836)
-    S.alpha_bin = (P.alg == ALG_HTSD || P.alg == ALG_HTCG) ? 0.0f : 0.25f;
+    S.alpha_bin = (P.alg == ALG_HTSD || P.alg == ALG_HTCG) ? 0.0f : P.timing ? P.alpha0 : 0.25f;
+    S.thr = 0.f;
     S.max_bin = (int)((float)P.num_bins * (1.0f - S.alpha_bin));
     S.min_val = 0.f;
     S.max_change = (P.alg == ALG_NIHT || P.alg == ALG_CGIHT) ? 1.0f : 0.0f;
@@ -1278,6 +1312,79 @@ __device__ void solve_instance(Team &t, Lds &L, const XProblem &P, int budget) {
       S.err = (float)sqrt(e2);
       push16(S.prev, S.err);
       stop_update(S, 749);
+    } else if ((P.alg == ALG_NIHT || P.alg == ALG_HTP) && P.timing) {
+      // NIHT_S_timings_gen / HTP_S_timings_gen (algorithms_timings.cu:546-640,
+      // 1359-1545): the support by supportFlag 0 (the skip rule), 1 (FindSupportSet
+      // asked to bin at every step; slope and intercept refreshed only when the skip
+      // rule allows), 2 (the sort-based threshold T, lowered by maxChange between
+      // sorts) or 3 (a sort at every step)
+      const unsigned long long c0 = wall_clock64();
+      float xm;
+      S.max_change = step_restricted_sd(t, L, P, S, &xm);
+      const unsigned long long c1 = wall_clock64();
+      const int sf = P.supp_flag;
+      if (sf < 2) {
+        if (S.min_val <= S.max_change) {
+          double v[1] = {xm};
+          const int op[1] = {1};
+          reduce(t, v, op);
+          S.max_value = (float)v[0];
+          S.slope = (float)(P.num_bins - 1) / S.max_value;
+        }
+        const float mc = sf == 0 ? S.max_change : S.max_value;
+        if (S.min_val > mc) {
+          S.min_val = S.min_val - mc;
+        } else {
+          zero_counters(t, P);
+          t.sync();
+          find_support_bin(t, L, P, P.x, P.bins, S, S.slope, S.max_value, &S.k_bin);
+        }
+        threshold_x(t, P, S.k_bin);
+        t.sync();
+        S.supp_count = build_support(t, P, S.k_bin);
+      } else {
+        // FindSupportSet_sort (functions.cu:438-458) with threshold_and_support
+        // (kernels.cu:85-95): entries below T are cut and their bins set to 2; the
+        // sort branch first sets every bin to 1.  A kept entry may keep bin 2, so the
+        // operator's support list also takes the nonzeros of x.
+        S.k_bin = 1;
+        const float mcT = sf == 2 ? S.max_change : S.thr;
+        t.sync();  // x complete before it is counted
+        const bool sort = !(S.thr > mcT);
+        if (sort)
+          S.thr = team_kth_largest_abs(t, P, P.x, P.k);
+        else
+          S.thr = __fsub_rn(S.thr, mcT);
+        const float T = S.thr;
+        for (int i = t.gtid(); i < n; i += t.gsize()) {
+          if (sort) P.bins[i] = 1;
+          if (fabsf(P.x[i]) < T) {
+            P.x[i] = 0.f;
+            P.bins[i] = 2;
+          }
+        }
+        t.sync();
+        S.supp_count = build_support_if(t, P, [&](int i) { return P.bins[i] <= 1 || P.x[i] != 0.f; });
+      }
+      const unsigned long long c2 = wall_clock64();
+      int icg = 0;
+      if (P.alg == ALG_NIHT) {
+        S.err = S.prev[15];
+        stop_update_timing(S, 749);
+      } else {
+        const float errCG = cg_projection(t, L, P, S, false, 15, &icg);
+        S.err = sqrtf(errCG);
+        push16(S.prev, S.err);
+        stop_update_timing(S, 125);
+      }
+      const unsigned long long c3 = wall_clock64();
+      if (P.trace_ticks && t.rank == 0 && threadIdx.x == 0) {
+        XG(unsigned long long) tt = P.trace_ticks + 4 * (size_t)S.iter;
+        tt[0] = c3 - c0;
+        tt[1] = c2 - c1;
+        tt[2] = (unsigned long long)icg;
+        tt[3] = P.alg == ALG_HTP ? c3 - c2 : 0ull;
+      }
     } else if (P.alg == ALG_NIHT || P.alg == ALG_HTP) {
       float xm;
       S.max_change = step_restricted_sd(t, L, P, S, &xm);

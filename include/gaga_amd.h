@@ -57,6 +57,13 @@ typedef struct xcs_options {
   int block_size;            /* threads per workgroup; 0 = auto */
   float proj_frac_tol;       /* 'projFracTol' of CGIHTprojected (default 3.0,
                                 gaga_gen.cu:190); <= 0 selects the default */
+  int timing;                /* 'timing' on (1): NIHT and HTP run their timing variants
+                                (algorithms_timings.cu), with per-iteration times */
+  int supp_flag;             /* 'supportFlag' of the timing variants: 0 adaptive binning,
+                                1 binning at every step, 2 sort-based threshold with
+                                dynamic detection, 3 a sort at every step (gaga_gen.cu:168) */
+  float alpha;               /* 'alpha' (alpha_start) of the timing variants, in [0, 1);
+                                xcs_options_default sets 0.25 */
 } xcs_options;
 
 /* Fill *opts with the reference defaults for (alg, ens, n). */
@@ -106,6 +113,12 @@ int xcs_cs_dct(const char *alg, const float *y, const int *rows, int k, int m, i
 
 /* ---- resident plans: inputs generated into / kept in HBM, solves timed on device ---- */
 typedef struct xcs_plan xcs_plan;
+/* Per-iteration timings of the last solve of a plan run with timing on (NIHT, HTP):
+ * iteration time and support-set time in ms, HTP's CG iterations and CG time in ms
+ * (time_per_iteration, time_supp_set, cg_per_iteration, time_for_cg of
+ * algorithms_timings.cu:1540-1543), for the first len iterations; any may be NULL. */
+int xcs_plan_timings(xcs_plan *plan, float *iter_ms, float *supp_ms, int *cg_iters, float *cg_ms,
+                     int len);
 /* Generate one instance on the device (seed/instance from opts) and keep it. */
 int xcs_plan_create(const char *alg, const char *ens, int k, int m, int n, int p,
                     const xcs_options *opts, xcs_plan **plan);

@@ -92,6 +92,9 @@ typedef struct {
   int restart;      /* CGIHT restartFlag */
   int threads;      /* OpenMP threads for the operators (<=0: all) */
   float proj_frac_tol; /* CGIHTprojected projFracTol (<= 0: 3.0, gaga_gen.cu:190) */
+  int timing;       /* NIHT/HTP: the timing variants (algorithms_timings.cu) */
+  int supp_flag;    /* their supportFlag 0-3 */
+  float alpha;      /* their alpha_start (< 0: 0.25) */
 } orc_opts;
 
 typedef struct {
@@ -106,11 +109,14 @@ typedef struct {
 
 /* x_out[n]: in = ignored (start is x=0), out = recovered vector.
  * trace_kbin / trace_err (may be NULL): per outer iteration, length maxiter+1:
- * entry 0 is the initial k_bin / ||y||, entry t the state after iteration t. */
+ * entry 0 is the initial k_bin / ||y||, entry t the state after iteration t.
+ * trace_cg (may be NULL): HTP timing variant, CG iterations of iteration t at t. */
 int orc32_solve(int alg, const orc_matrix *M, const float *y, int k, const orc_opts *o,
-                float *x_out, orc_result *res, int *trace_kbin, float *trace_err);
+                float *x_out, orc_result *res, int *trace_kbin, float *trace_err,
+                int *trace_cg);
 int orc64_solve(int alg, const orc_matrix *M, const float *y, int k, const orc_opts *o,
-                double *x_out, orc_result *res, int *trace_kbin, double *trace_err);
+                double *x_out, orc_result *res, int *trace_kbin, double *trace_err,
+                int *trace_cg);
 
 /* The HTP/CSMPSP inner CG projection (algorithms.cu:2607-2669) on a fixed
  * support mask (mask[i] != 0: in S), from x = 0, at most 15 steps (the

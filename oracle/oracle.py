@@ -48,7 +48,8 @@ class Matrix(ctypes.Structure):
 class Opts(ctypes.Structure):
     _fields_ = [("tol", ctypes.c_float), ("maxiter", ctypes.c_int), ("num_bins", ctypes.c_int),
                 ("restart", ctypes.c_int), ("threads", ctypes.c_int),
-                ("proj_frac_tol", ctypes.c_float)]
+                ("proj_frac_tol", ctypes.c_float), ("timing", ctypes.c_int),
+                ("supp_flag", ctypes.c_int), ("alpha", ctypes.c_float)]
 
 
 class Result(ctypes.Structure):
@@ -78,8 +79,8 @@ def _declare(L):
     L.orc_apply_AT64.argtypes = [_p, _p, _p]
     L.orc_dct2_ortho.argtypes = [_i, _p, _p]
     L.orc_dct3_ortho.argtypes = [_i, _p, _p]
-    L.orc32_solve.argtypes = [_i, _p, _p, _i, _p, _p, _p, _p, _p]
-    L.orc64_solve.argtypes = [_i, _p, _p, _i, _p, _p, _p, _p, _p]
+    L.orc32_solve.argtypes = [_i, _p, _p, _i, _p, _p, _p, _p, _p, _p]
+    L.orc64_solve.argtypes = [_i, _p, _p, _i, _p, _p, _p, _p, _p, _p]
     L.orc_results.argtypes = [_i, _i, _p, _p, _p, _p]
     L.orc64_cg_fixed.argtypes = [_p, _p, _p, _f, _p]
     L.orc64_cg_fixed.restype = _i
@@ -215,12 +216,12 @@ def make_problem(ens, m, n, seed, inst, p=7, smv_ens=2, lda=None):
 
 
 def solve(alg, prob, y, k, tol=1e-5, maxiter=None, num_bins=0, restart=0, precision=64,
-          trace=False, proj_frac_tol=3.0):
+          trace=False, proj_frac_tol=3.0, timing=False, supp_flag=0, alpha=-1.0, trace_cg=None):
     """Run the oracle solver.  Returns (x, Result, trace_kbin, trace_err)."""
     if maxiter is None:
         maxiter = 300 if alg in ("HTP", "CSMPSP") else 5000
     o = Opts(tol=tol, maxiter=maxiter, num_bins=num_bins, restart=restart, threads=0,
-             proj_frac_tol=proj_frac_tol)
+             proj_frac_tol=proj_frac_tol, timing=int(timing), supp_flag=supp_flag, alpha=alpha)
     y = np.ascontiguousarray(y, dtype=np.float32)
     res = Result()
     tk = np.full(maxiter + 2, -1, dtype=np.int32) if trace else None
@@ -228,12 +229,12 @@ def solve(alg, prob, y, k, tol=1e-5, maxiter=None, num_bins=0, restart=0, precis
         x = np.zeros(prob.n, dtype=np.float64)
         te = np.zeros(maxiter + 2, dtype=np.float64) if trace else None
         lib().orc64_solve(ALGS[alg], ctypes.byref(prob.M), _ptr(y), k, ctypes.byref(o), _ptr(x),
-                          ctypes.byref(res), _ptr(tk), _ptr(te))
+                          ctypes.byref(res), _ptr(tk), _ptr(te), _ptr(trace_cg))
     else:
         x = np.zeros(prob.n, dtype=np.float32)
         te = np.zeros(maxiter + 2, dtype=np.float32) if trace else None
         lib().orc32_solve(ALGS[alg], ctypes.byref(prob.M), _ptr(y), k, ctypes.byref(o), _ptr(x),
-                          ctypes.byref(res), _ptr(tk), _ptr(te))
+                          ctypes.byref(res), _ptr(tk), _ptr(te), _ptr(trace_cg))
     if trace:
         nt = res.iters + 1
         return x, res, tk[:nt], te[:nt]

@@ -104,7 +104,7 @@ static int F(find_support_set)(const REAL *v, int *bins, int *counters, REAL slo
     k_bin = k_bin - 1;
     if (sum < k) {
       *alpha = *alpha * (REAL)0.5;
-      *max_bin = (int)(num_bins * (1 - *alpha));
+      *max_bin = (int)((float)num_bins * (1.0f - (float)*alpha));
     }
     minVal = intercept - (REAL)(k_bin + 1) / slope;
   }
@@ -281,7 +281,8 @@ static REAL F(cg_unrestricted)(F(ws) *w, REAL *mu, int beta_to_zero, int k_bin) 
 
 /* Inner CG projection used by HTP and CSMPSP (algorithms.cu:2607-2669,
  * 3548-3604).  Returns errCG (squared residual norm). */
-static REAL F(cg_projection)(F(ws) *w, int k_bin, REAL tolCG, REAL tol2, int fixed_diff) {
+static REAL F(cg_projection)(F(ws) *w, int k_bin, REAL tolCG, REAL tol2, int fixed_diff,
+                             int *iters) {
   int m = w->m, n = w->n;
   REAL hist[16], evo[16];
   int iterCG = 0;
@@ -308,6 +309,7 @@ static REAL F(cg_projection)(F(ws) *w, int k_bin, REAL tolCG, REAL tol2, int fix
     evo[15] = hist[14] - hist[15];
     diffCG = fixed_diff ? (REAL)10 : F(max_list)(evo, 16);
   }
+  if (iters) *iters = iterCG;
   return errCG;
 }
 
@@ -343,6 +345,48 @@ static void F(stop_update)(F(stop) *s, const REAL *prev, int iter, int fail_afte
     rate = POW(rate, root);
     if (rate > (REAL)0.999f) s->fail = 1;
   }
+}
+
+/* The timing variants' bookkeeping (algorithms_timings.cu:607-619, 1525-1539): no
+ * cycle test, the stagnation rate over the ring with root 1/16. */
+static void F(stop_update_timing)(F(stop) *s, const REAL *prev, int iter, int fail_after) {
+  for (int j = 0; j < 15; j++) s->evo[j] = s->evo[j + 1];
+  s->evo[15] = prev[14] - prev[15];
+  s->diff = F(max_list)(s->evo, 16);
+  if (iter > fail_after) {
+    REAL root = (REAL)(1 / 16.0f);
+    REAL rate = prev[15] / prev[0];
+    rate = POW(rate, root);
+    if (rate > (REAL)0.999f) s->fail = 1;
+  }
+}
+
+static int F(cmp_real)(const void *a, const void *b) {
+  const REAL x = *(const REAL *)a, y = *(const REAL *)b;
+  return (x > y) - (x < y);
+}
+
+/* FindSupportSet_sort (functions.cu:438-458) with threshold_and_support
+ * (kernels.cu:85-95): while the running threshold T exceeds maxChange it is
+ * lowered by maxChange and the entries below it are cut (their bins set to 2,
+ * the other bins left as they are); otherwise T is the k-th largest |x| (a full
+ * sort), every bin is set to 1 and the entries below T are cut. */
+static REAL F(sort_support)(REAL *x, int *bins, REAL T, REAL maxChange, int n, int k,
+                            REAL *scratch) {
+  if (T > maxChange) {
+    T = T - maxChange;
+  } else {
+    for (int i = 0; i < n; i++) scratch[i] = FABS(x[i]);
+    qsort(scratch, (size_t)n, sizeof(REAL), F(cmp_real));
+    T = scratch[n - k];
+    for (int i = 0; i < n; i++) bins[i] = 1;
+  }
+  for (int i = 0; i < n; i++)
+    if (FABS(x[i]) < T) {
+      x[i] = 0;
+      bins[i] = 2;
+    }
+  return T;
 }
 
 static void F(trace)(int *tk, REAL *te, int idx, int k_bin, REAL e) {
@@ -424,7 +468,7 @@ void orc_stop_update(const float prev[16], float evo[16], int iter, int fail_aft
 #endif
 
 int F(solve)(int alg, const orc_matrix *M, const float *y_in, int k, const orc_opts *o,
-             REAL *x_out, orc_result *res, int *trace_kbin, REAL *trace_err) {
+             REAL *x_out, orc_result *res, int *trace_kbin, REAL *trace_err, int *trace_cg) {
   int m = M->m, n = M->n;
   int num_bins = o->num_bins > 0 ? o->num_bins : (n / 20 > 1000 ? n / 20 : 1000);
   REAL tol = (REAL)o->tol;
@@ -454,7 +498,15 @@ int F(solve)(int alg, const orc_matrix *M, const float *y_in, int k, const orc_o
 
   int iter = 0, k_bin = 0, k_bin_grad = 0, sum = 0;
   REAL alpha = (REAL)0.25f;
-  int MaxBin = (int)(num_bins * (1 - alpha));
+  int MaxBin = (int)((float)num_bins * (1.0f - (float)alpha));
+  /* the timing variants NIHT_S_timings_* / HTP_S_timings_* (algorithms_timings.cu:
+   * 453-661, 1266-1563) take alpha and supportFlag as options */
+  const int timing = o->timing && (alg == ORC_NIHT || alg == ORC_HTP);
+  const int supp_flag = timing ? o->supp_flag : 0;
+  if (timing && o->alpha >= 0) {
+    alpha = (REAL)o->alpha;
+    MaxBin = (int)((float)num_bins * (1.0f - (float)alpha));
+  }
   REAL minVal = 0;
   REAL maxChange = (alg == ORC_NIHT || alg == ORC_CGIHT) ? (REAL)1 : (REAL)0;
   REAL mu = 0;
@@ -581,7 +633,45 @@ int F(solve)(int alg, const orc_matrix *M, const float *y_in, int k, const orc_o
   F(threshold)(n, w->x, w->bins, k_bin);
   F(trace)(trace_kbin, trace_err, 0, k_bin, S.err);
 
-  if (alg == ORC_NIHT) {
+  if (timing) {
+    /* NIHT_S_timings_gen / HTP_S_timings_gen loops (algorithms_timings.cu:546-640,
+     * 1359-1545): the support by supportFlag 0 (the skip rule), 1 (binning at every
+     * step, slope and intercept refreshed only when the skip rule allows), 2 (the
+     * sort-based threshold lowered by maxChange between sorts) or 3 (a sort at
+     * every step); HTP adds its CG projection (iterations into trace_cg). */
+    REAL T = 0;
+    REAL *scratch = (REAL *)malloc(sizeof(REAL) * nn);
+    while (F(should_continue)(&S, tol, iter, maxiter, &reason)) {
+      maxChange = F(restricted_sd)(w, k_bin);
+      if (supp_flag < 2 && minVal <= maxChange) {
+        max_value = F(maxmag)(n, w->x);
+        slope = ((REAL)(num_bins - 1) / (max_value));
+      }
+      if (supp_flag == 0 || supp_flag == 1) {
+        sum = F(find_support_set)(w->x, w->bins, w->counters, slope, max_value,
+                                  supp_flag == 0 ? maxChange : max_value, &minVal, &alpha,
+                                  &MaxBin, &k_bin, n, k, num_bins, sum);
+        F(threshold)(n, w->x, w->bins, k_bin);
+      } else {
+        k_bin = 1;
+        T = F(sort_support)(w->x, w->bins, T, supp_flag == 2 ? maxChange : T, n, k, scratch);
+      }
+      if (alg == ORC_HTP) {
+        int icg = 0;
+        REAL errCG = F(cg_projection)(w, k_bin, tolCG, tol2, 0, &icg);
+        if (trace_cg) trace_cg[iter] = icg;
+        S.err = SQRT(errCG);
+        F(push)(w->residNorm_prev, S.err);
+        F(stop_update_timing)(&S, w->residNorm_prev, iter, 125);
+      } else {
+        S.err = w->residNorm_prev[15];
+        F(stop_update_timing)(&S, w->residNorm_prev, iter, 749);
+      }
+      iter++;
+      F(trace)(trace_kbin, trace_err, iter, k_bin, w->residNorm_prev[15]);
+    }
+    free(scratch);
+  } else if (alg == ORC_NIHT) {
     /* NIHT_S_gen (algorithms.cu:1274-1441) */
     while (F(should_continue)(&S, tol, iter, maxiter, &reason)) {
       maxChange = F(restricted_sd)(w, k_bin);
@@ -608,7 +698,7 @@ int F(solve)(int alg, const orc_matrix *M, const float *y_in, int k, const orc_o
       sum = F(find_support_set)(w->x, w->bins, w->counters, slope, max_value, maxChange,
                                 &minVal, &alpha, &MaxBin, &k_bin, n, k, num_bins, sum);
       F(threshold)(n, w->x, w->bins, k_bin);
-      REAL errCG = F(cg_projection)(w, k_bin, tolCG, tol2, 0);
+      REAL errCG = F(cg_projection)(w, k_bin, tolCG, tol2, 0, NULL);
       S.err = SQRT(errCG);
       F(push)(w->residNorm_prev, S.err);
       F(stop_update)(&S, w->residNorm_prev, iter, 125);
@@ -617,7 +707,7 @@ int F(solve)(int alg, const orc_matrix *M, const float *y_in, int k, const orc_o
     }
   } else if (alg == ORC_CSMPSP) {
     /* CSMPSP_S_gen (algorithms.cu:3372-3674) */
-    REAL errCG = F(cg_projection)(w, k_bin, tolCG, tol2, 0);
+    REAL errCG = F(cg_projection)(w, k_bin, tolCG, tol2, 0, NULL);
     S.err = SQRT(errCG);
     w->residNorm_prev[15] = S.err;
     iter++;
@@ -632,7 +722,7 @@ int F(solve)(int alg, const orc_matrix *M, const float *y_in, int k, const orc_o
       /* joinSupports (kernels.cu:162-172) */
       for (int i = 0; i < n; i++)
         if (w->bins_grad[i] <= k_bin_grad && w->bins[i] > k_bin) w->bins[i] = k_bin;
-      F(cg_projection)(w, k_bin, tolCG, tol2, 1);
+      F(cg_projection)(w, k_bin, tolCG, tol2, 1, NULL);
       max_value = F(maxmag)(n, w->x);
       minVal = 0;
       slope = ((REAL)(num_bins - 1) / (max_value));

@@ -1,0 +1,99 @@
+"""NIHT and HTP timing variants (algorithms_timings.cu) on the GPU against the oracle:
+every supportFlag on gen, smv and dct (same support as the FP32 oracle, the FP64
+oracle's solution to 1e-5, iterations within 10 %), a problem where the plain loop
+would stop by its cycle test, the per-iteration timings, and batch = plan."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SEED = 20260101
+
+
+def _problem(O, ens, m, n, k, inst):
+    P = O.make_problem(ens, m, n, SEED, inst)
+    x = O.sparse_vector(n, k, "binary", SEED, inst)
+    y = P.apply_A(x).astype(np.float32)
+    return P, x, y
+
+
+def _solve_dev(alg, ens, P, y, k, **kw):
+    import gaga_amd as ga
+    opts = ga.options(tol=1e-5, timing="on", **kw)
+    if ens == "gen":
+        return ga.cs(alg, "gen", y, P.A.T[:P.m], k, opts)
+    if ens == "smv":
+        cols = np.repeat(np.arange(P.n, dtype=np.int32), np.diff(P.col_ptr))
+        return ga.cs(alg, "smv", y, P.rows + 1, cols + 1, P.vals, k, opts)
+    return ga.cs(alg, "dct", y, P.dct_rows + 1, k, P.m, P.n, opts)
+
+
+@pytest.mark.parametrize("alg", ["NIHT", "HTP"])
+@pytest.mark.parametrize("sf", [0, 1, 2, 3])
+@pytest.mark.parametrize("ens,m,n", [("gen", 256, 1024), ("smv", 256, 2048), ("dct", 256, 1024)])
+def test_matches_oracle(alg, sf, ens, m, n, oracle):
+    for inst in range(2):
+        print("case", alg, "supportFlag", sf, ens, m, n, "instance", inst, flush=True)
+        P, x, y = _problem(oracle, ens, m, n, 20, inst)
+        xd, it, _ = _solve_dev(alg, ens, P, y, 20, supportFlag=sf)
+        x32, r32, _, _ = oracle.solve(alg, P, y, 20, tol=1e-5, precision=32, timing=True,
+                                      supp_flag=sf)
+        x64, r64, _, _ = oracle.solve(alg, P, y, 20, tol=1e-5, precision=64, timing=True,
+                                      supp_flag=sf)
+        np.testing.assert_array_equal(xd != 0, x32 != 0)
+        assert np.linalg.norm(xd - x64) <= 1e-5 * np.linalg.norm(x64)
+        assert abs(it - r32.iters) <= max(2, r32.iters // 10)
+        assert np.max(np.abs(xd - x)) < 1e-4
+
+
+def test_runs_past_the_plain_cycle_stop(oracle):
+    """gen 128 x 512, k = 30, instance 4: plain HTP stops by its cycle test after 10
+    iterations; the timing variant has no cycle test and goes on, on the device as in
+    the oracle."""
+    P, _, y = _problem(oracle, "gen", 128, 512, 30, 4)
+    xd, it, _ = _solve_dev("HTP", "gen", P, y, 30, supportFlag=0)
+    x64, r64, _, _ = oracle.solve("HTP", P, y, 30, tol=1e-5, precision=64, timing=True, supp_flag=0)
+    assert r64.iters > 10
+    assert abs(it - r64.iters) <= max(3, r64.iters // 5)
+
+
+@pytest.mark.parametrize("alg", ["NIHT", "HTP"])
+def test_timings(alg):
+    import gaga_amd as ga
+    pl = ga.Plan(alg, "gen", 20, 256, 1024, 0,
+                 ga.options(tol=1e-5, seed=SEED, vecDistribution="binary", instance=3, timing="on",
+                            supportFlag=3))
+    ms = pl.solve()
+    r = pl.results()
+    n_it = int(r["iters"])
+    it_ms, supp_ms, cg, cg_ms = pl.timings(n_it)
+    assert n_it > 0 and r["norms"][2] < 1e-3
+    assert np.all(it_ms > 0) and np.all(supp_ms > 0) and np.all(supp_ms <= it_ms)
+    assert it_ms.sum() <= ms * 1.01
+    if alg == "HTP":
+        assert np.all((cg >= 1) & (cg <= 15)) and np.all(cg_ms > 0)
+    else:
+        assert np.all(cg == 0)
+    pl.close()
+
+
+def test_batch_equals_plans():
+    import gaga_amd as ga
+    specs = [dict(ens="gen", k=20, m=256, n=1024, instance=0),
+             dict(ens="smv", k=12, m=128, n=1024, p=7, instance=1),
+             dict(ens="dct", k=20, m=256, n=1024, instance=2)]
+    base = dict(tol=1e-5, seed=SEED, vecDistribution="binary", timing="on", supportFlag=2)
+    B = ga.Batch("NIHT", specs, ga.options(**base))
+    B.solve()
+    rb = B.results()
+    for i, s in enumerate(specs):
+        extra = {"matrixEnsemble": "binary"} if s["ens"] == "smv" else {}
+        pl = ga.Plan("NIHT", s["ens"], s["k"], s["m"], s["n"], s.get("p", 0),
+                     ga.options(instance=s["instance"], teamSize=1, blockSize=256, **base, **extra))
+        pl.solve()
+        r = pl.results()
+        xb, _ = B.x(i)
+        np.testing.assert_array_equal(xb, r["x"])
+        assert rb["iters"][i] == r["iters"]
+        pl.close()
+    B.close()

@@ -1,0 +1,110 @@
+"""The oracle's NIHT and HTP timing variants (NIHT_S_timings_gen, HTP_S_timings_gen,
+algorithms_timings.cu:453-661, 1266-1563) pinned by what fixes them:
+
+- supportFlag 0 selects the support as the plain loops do, so where the plain loop
+  stops by its residual the iterates are identical, and where the plain loop stops by
+  its cycle test the timing loop (which has none) repeats its iterates up to there
+  and then goes on;
+- supportFlag 3 (a sort at every step): each iterate is the k largest entries of the
+  steepest-descent point x + mu g, mu the exact line search along the gradient
+  restricted to the previous support (numpy.linalg.lstsq);
+- supportFlag 1 and 2: each iterate is a top set of that point, its values kept;
+- every flag recovers the planted signals of the parity problems in both precisions.
+"""
+import numpy as np
+import pytest
+
+SEED = 20260101
+
+
+def _problem(O, ens, m, n, k, inst):
+    P = O.make_problem(ens, m, n, SEED, inst)
+    x = O.sparse_vector(n, k, "binary", SEED, inst)
+    y = P.apply_A(x).astype(np.float32)
+    return P, x, y
+
+
+@pytest.mark.parametrize("alg", ["NIHT", "HTP"])
+@pytest.mark.parametrize("ens,m,n", [("gen", 256, 1024), ("smv", 256, 2048), ("dct", 256, 1024)])
+def test_flag0_equals_plain_loop(oracle, alg, ens, m, n):
+    for inst in range(2):
+        P, _, y = _problem(oracle, ens, m, n, 20, inst)
+        xp, rp, _, _ = oracle.solve(alg, P, y, 20, tol=1e-5, precision=64)
+        xt, rt, _, _ = oracle.solve(alg, P, y, 20, tol=1e-5, precision=64, timing=True,
+                                    supp_flag=0)
+        assert rp.stop_reason == 0 and rt.stop_reason == 0
+        assert rt.iters == rp.iters
+        np.testing.assert_array_equal(xt, xp)
+
+
+@pytest.mark.parametrize("alg,inst,plain_iters", [("HTP", 4, 10), ("NIHT", 5, 119)])
+def test_no_cycle_test(oracle, alg, inst, plain_iters):
+    """gen 128 x 512, k = 30: the plain loop ends by the cycle test; the timing loop
+    has the same iterates up to there and continues (NIHT until the stagnation
+    rule fires after iteration 749)."""
+    P, _, y = _problem(oracle, "gen", 128, 512, 30, inst)
+    xp, rp, _, _ = oracle.solve(alg, P, y, 30, tol=1e-5, precision=64)
+    assert rp.stop_reason == 5 and rp.iters == plain_iters
+    xs, rs, _, _ = oracle.solve(alg, P, y, 30, tol=1e-5, precision=64, timing=True, supp_flag=0,
+                                maxiter=plain_iters)
+    np.testing.assert_array_equal(xs, xp)
+    _, rt, _, _ = oracle.solve(alg, P, y, 30, tol=1e-5, precision=64, timing=True, supp_flag=0)
+    assert rt.stop_reason != 5 and rt.iters > plain_iters
+    if alg == "NIHT":
+        assert rt.stop_reason == 4 and rt.iters == 751
+
+
+def _sd_point(P, x_prev, y64, n):
+    S = np.flatnonzero(x_prev)
+    r = y64 - P.apply_A(x_prev)
+    g = P.apply_AT(r)
+    gS = np.zeros(n)
+    gS[S] = g[S]
+    d = P.apply_A(gS)
+    mu = np.linalg.lstsq(d[:, None], r, rcond=None)[0][0]
+    return x_prev + mu * g
+
+
+@pytest.mark.parametrize("ens,m,n,k,sf", [("gen", 256, 1024, 20, 3), ("dct", 256, 1024, 20, 3),
+                                          ("gen", 128, 512, 40, 3), ("gen", 256, 1024, 20, 1),
+                                          ("gen", 128, 512, 40, 1), ("gen", 256, 1024, 20, 2),
+                                          ("gen", 128, 512, 40, 2), ("dct", 256, 1024, 20, 2)])
+def test_iterates(oracle, ens, m, n, k, sf):
+    P, _, y = _problem(oracle, ens, m, n, k, 1)
+    y64 = y.astype(np.float64)
+    x_prev, _, _, _ = oracle.solve("NIHT", P, y, k, tol=1e-5, maxiter=0, precision=64,
+                                   timing=True, supp_flag=sf)
+    for j in range(1, 6):
+        xj, rj, _, _ = oracle.solve("NIHT", P, y, k, tol=1e-5, maxiter=j, precision=64,
+                                    timing=True, supp_flag=sf)
+        if rj.iters < j:
+            break
+        z = _sd_point(P, x_prev, y64, n)
+        Sj = np.flatnonzero(xj)
+        np.testing.assert_allclose(xj[Sj], z[Sj], rtol=1e-9, atol=1e-12 * np.max(np.abs(z)))
+        # a top set of z
+        assert np.min(np.abs(z[Sj])) >= np.max(np.abs(np.delete(z, Sj))) * (1 - 1e-9)
+        if sf == 3:
+            assert set(Sj) == set(np.argsort(-np.abs(z))[:k])
+        x_prev = xj
+
+
+@pytest.mark.parametrize("alg", ["NIHT", "HTP"])
+@pytest.mark.parametrize("sf", [0, 1, 2, 3])
+def test_recovers(oracle, alg, sf):
+    for ens, m, n in [("gen", 256, 1024), ("smv", 256, 2048), ("dct", 256, 1024)]:
+        P, x, y = _problem(oracle, ens, m, n, 20, 2)
+        for prec in (32, 64):
+            xo, res, _, _ = oracle.solve(alg, P, y, 20, tol=1e-5, precision=prec, timing=True,
+                                         supp_flag=sf)
+            assert res.stop_reason == 0
+            assert np.max(np.abs(xo - x)) < 1e-4
+
+
+def test_alpha_sets_the_bin_range(oracle):
+    """alpha 0 (every bin counted from the start), 0.25 and 0.5 all recover."""
+    P, x, y = _problem(oracle, "gen", 256, 1024, 20, 0)
+    for a in (0.0, 0.25, 0.5):
+        xo, res, _, _ = oracle.solve("NIHT", P, y, 20, tol=1e-5, precision=64, timing=True,
+                                     supp_flag=0, alpha=a)
+        assert res.stop_reason == 0 and np.max(np.abs(xo - x)) < 1e-4
