@@ -54,6 +54,8 @@ def options(*args, **kwargs):
 
 
 def _apply_options(alg, ens, n, opts):
+    if isinstance(opts, _capi.Options):
+        return opts
     o = _capi.default_options(alg, ens, n)
     for name, val in (opts or []):
         if name == "tol":
@@ -194,6 +196,22 @@ def _generate(alg, ens, k, m, n, p, o, results_dir):
     reference always writes to its working directory) the reference's results
     line is appended to resultsDir/gpu_data_<ens>YYYYMMDD.txt, as
     results.cu:509-525 does after every generated solve."""
+    if results_dir is not None and o.timing and alg in ("NIHT", "HTP"):
+        # the timing variants write their own text to gpu_timings_data_<ens>YYYYMMDD.txt
+        # (results_timings_gen and twins, results.cu:1395-1560)
+        pl = _capi.Plan(alg, ens, k, m, n, p, o)
+        try:
+            pl.solve()
+            r = pl.results()
+            times = pl.times()
+            it = int(r["iters"])
+            t_iter, t_supp, cg, t_cg = pl.timings(it)
+            _capi.results_timing_append(results_dir, alg, ens, k, m, n, p, o, r["norms"], times, it,
+                                        r["support"], r["conv_rate"], o.alpha, o.supp_flag,
+                                        t_iter, t_supp, cg, t_cg)
+            return r["norms"], times, it, r["support"], r["conv_rate"], r["x"]
+        finally:
+            pl.close()
     out = _capi.generate(alg, ens, k, m, n, p, o)
     if results_dir is not None:
         norms, times, iters, sup, rate, _ = out

@@ -11,6 +11,7 @@ LIB_PATH = os.environ.get("XCS_LIB_PATH", os.path.join(_HERE, "libgaga_amd.so"))
 EXPORTS = (
     "xcs_options_default", "xcs_cs_generate", "xcs_cs_gen", "xcs_cs_smv", "xcs_cs_dct",
     "xcs_plan_create", "xcs_plan_solve", "xcs_plan_results", "xcs_plan_trace", "xcs_plan_timings",
+    "xcs_plan_times", "xcs_results_timing_text", "xcs_results_timing_append",
     "xcs_plan_destroy", "xcs_plan_bytes", "xcs_plan_apply_timed",
     "xcs_batch_create", "xcs_batch_bytes", "xcs_batch_apply_timed", "xcs_batch_solve",
     "xcs_batch_results", "xcs_batch_configure", "xcs_batch_tune",
@@ -73,6 +74,11 @@ def load():
         L.xcs_plan_results.argtypes = [_P, _P, _P, _P, _P, _P, _P, _P]
         L.xcs_plan_trace.argtypes = [_P, _P, _P, _I]
         L.xcs_plan_timings.argtypes = [_P, _P, _P, _P, _P, _I]
+        L.xcs_plan_times.argtypes = [_P, _P]
+        _tline = [_I, _I, _I, _I, _I, _I, _P, _P, _I, ctypes.c_float, _P, ctypes.c_uint,
+                  ctypes.c_float, _I, _P, _P, _P, _P]
+        L.xcs_results_timing_text.argtypes = [_P, ctypes.c_size_t, _P, _P] + _tline
+        L.xcs_results_timing_append.argtypes = [_P, _P, _P] + _tline
         L.xcs_plan_destroy.argtypes = [_P]
         L.xcs_plan_destroy.restype = None
         L.xcs_plan_bytes.argtypes = [_P, _P]
@@ -177,6 +183,37 @@ def results_append(directory, alg, ens, k, m, n, p, o, norms, times, iters, sup,
     """Append the results line to <directory>/gpu_data_<ens>YYYYMMDD.txt (results.cu:509-525)."""
     args, keep = _line_args(alg, ens, k, m, n, p, o, norms, times, iters, sup, rate)
     _ck(load().xcs_results_append(str(directory).encode(), *args))
+
+
+def _timing_args(alg, ens, k, m, n, p, o, norms, times, iters, sup, rate, alpha, supp_flag,
+                 t_iter, t_supp, cg, t_cg):
+    args, keep = _line_args(alg, ens, k, m, n, p, o, norms, times, iters, sup, rate)
+    arrs = [np.ascontiguousarray(t_iter, np.float32), np.ascontiguousarray(t_supp, np.float32),
+            None if cg is None else np.ascontiguousarray(cg, np.int32),
+            None if t_cg is None else np.ascontiguousarray(t_cg, np.float32)]
+    return args[2:] + (float(alpha), int(supp_flag)) + tuple(_p(a) if a is not None else None
+                                                             for a in arrs), (keep, arrs, args)
+
+
+def results_timing_text(alg, ens, k, m, n, p, o, norms, times, iters, sup, rate, alpha, supp_flag,
+                        t_iter, t_supp, cg=None, t_cg=None):
+    """The timing variants' results text (File_output_timings*, functions.cu:82-212)."""
+    rest, keep = _timing_args(alg, ens, k, m, n, p, o, norms, times, iters, sup, rate, alpha,
+                              supp_flag, t_iter, t_supp, cg, t_cg)
+    size = 1024 + 160 * max(int(iters), 0)
+    buf = ctypes.create_string_buffer(size)
+    r = load().xcs_results_timing_text(buf, size, alg.encode(), ens.encode(), *rest)
+    if r < 0:
+        _ck(r)
+    return buf.value.decode()
+
+
+def results_timing_append(directory, alg, ens, k, m, n, p, o, norms, times, iters, sup, rate,
+                          alpha, supp_flag, t_iter, t_supp, cg=None, t_cg=None):
+    """Append it to <directory>/gpu_timings_data_<ens>YYYYMMDD.txt (results.cu:1540-1560)."""
+    rest, keep = _timing_args(alg, ens, k, m, n, p, o, norms, times, iters, sup, rate, alpha,
+                              supp_flag, t_iter, t_supp, cg, t_cg)
+    _ck(load().xcs_results_timing_append(str(directory).encode(), alg.encode(), ens.encode(), *rest))
 
 
 def solve_gen(alg, y, A, k, o):
@@ -322,6 +359,12 @@ class Plan:
         er = np.zeros(length, np.float32)
         _ck(load().xcs_plan_trace(self._h, _p(kb), _p(er), length))
         return kb, er
+
+    def times(self):
+        """(solve ms, ms per iteration, solve + generation ms) of the last solve."""
+        t = np.zeros(3, np.float32)
+        _ck(load().xcs_plan_times(self._h, _p(t)))
+        return t
 
     def timings(self, length):
         """Per-iteration (iteration ms, support-set ms, CG iterations, CG ms) of the last

@@ -1502,6 +1502,117 @@ static int results_line(std::string &out, const char *alg, int ens, int k, int m
   return XCS_OK;
 }
 
+// The timing variants' results text (File_output_timings*, functions.cu:82-212):
+// the line with alpha and supp_flag (smv: NonzerosPerCol and matrixEnsemble at its
+// end, gen: matrixEnsemble), then one line per iteration (HTP adds its CG steps and
+// CG time).
+static int results_timing_text(std::string &out, const char *alg, int a, int ens, int k, int m,
+                               int n, int p, int mat_ens, int vec_dist, const float *e,
+                               const float *t, int iters, float rate, const int *sp,
+                               unsigned int seed, float alpha, int supp_flag, const float *ti,
+                               const float *ts, const int *cg, const float *tcg) {
+  const char *tag = ens == ENS_GEN ? "gen" : ens == ENS_SMV ? "smv" : "dct";
+  char b[1024];
+  int w = snprintf(b, sizeof(b),
+                   "%s_S_%s output: k %d, m %d, n %d, vecDistribution %d "
+                   "errorl1 %0.7e errorl2 %0.7e errorlinf %0.7e timeALG %0.7e timeIter %0.7e "
+                   "timeTotal %0.7e iterations %d converge_rate %0.7e TruePos %d FalsePos %d "
+                   "TrueNeg %d FalseNeg %d seed %u alpha %0.7e ",
+                   alg, tag, k, m, n, vec_dist, e[0], e[1], e[2], t[0], t[1], t[2], iters, rate,
+                   sp[0], sp[1], sp[2], sp[3], seed, alpha);
+  if (ens == ENS_SMV)
+    w += snprintf(b + w, sizeof(b) - w, "supp_flag %d NonzerosPerCol %d matrixEnsemble %d \n",
+                  supp_flag, p, mat_ens);
+  else if (ens == ENS_GEN)
+    w += snprintf(b + w, sizeof(b) - w, "supp_flag %d matrixEnsemble %d \n", supp_flag, mat_ens);
+  else
+    w += snprintf(b + w, sizeof(b) - w, "supp_flag %d \n", supp_flag);
+  if (w <= 0 || w >= (int)sizeof(b)) return set_err(XCS_EINVAL, "results line too long");
+  out.assign(b, (size_t)w);
+  for (int j = 0; j < iters; j++) {
+    char l[256];
+    int lw;
+    if (a == ALG_HTP)
+      lw = snprintf(l, sizeof(l),
+                    "iteration %d time_iteration %0.7e time_supp_set %0.7e cg_per_iteration %0.0f "
+                    "time_for_cg %0.7e\n",
+                    j, ti[j], ts[j], (float)cg[j], tcg[j]);
+    else
+      lw = snprintf(l, sizeof(l), "iteration %d time_iteration %0.7e time_supp_set %0.7e\n", j,
+                    ti[j], ts[j]);
+    out.append(l, (size_t)lw);
+  }
+  return XCS_OK;
+}
+
+static int timing_args_ok(int a, int iters, const float *ti, const float *ts, const int *cg,
+                          const float *tcg) {
+  if (a != ALG_NIHT && a != ALG_HTP) return set_err(XCS_EINVAL, "timing results exist for NIHT and HTP");
+  if (iters < 0) return set_err(XCS_EINVAL, "iterations < 0");
+  if (iters > 0 && (!ti || !ts || (a == ALG_HTP && (!cg || !tcg))))
+    return set_err(XCS_EINVAL, "NULL per-iteration array");
+  return XCS_OK;
+}
+
+int xcs_results_timing_text(char *buf, size_t buf_len, const char *alg, const char *ens, int k,
+                            int m, int n, int p, int matrix_ensemble, int vec_distribution,
+                            const float norms[3], const float times[3], int iterations,
+                            float conv_rate, const int support[4], unsigned int seed, float alpha,
+                            int supp_flag, const float *time_iteration, const float *time_supp_set,
+                            const int *cg_per_iteration, const float *time_for_cg) {
+  const int a = parse_alg(alg), e = parse_ens(ens);
+  if (a < 0 || e < 0) return set_err(XCS_EINVAL, "unknown algorithm or ensemble");
+  if (!buf || !norms || !times || !support) return set_err(XCS_EINVAL, "NULL argument");
+  XCS_TRY(timing_args_ok(a, iterations, time_iteration, time_supp_set, cg_per_iteration, time_for_cg));
+  std::string text;
+  XCS_TRY(results_timing_text(text, alg, a, e, k, m, n, p, matrix_ensemble, vec_distribution, norms,
+                              times, iterations, conv_rate, support, seed, alpha, supp_flag,
+                              time_iteration, time_supp_set, cg_per_iteration, time_for_cg));
+  if (text.size() + 1 > buf_len)
+    return set_err(XCS_EINVAL, "buffer of %zu bytes holds no %zu-byte text", buf_len, text.size() + 1);
+  memcpy(buf, text.c_str(), text.size() + 1);
+  return (int)text.size();
+}
+
+int xcs_results_timing_append(const char *dir, const char *alg, const char *ens, int k, int m,
+                              int n, int p, int matrix_ensemble, int vec_distribution,
+                              const float norms[3], const float times[3], int iterations,
+                              float conv_rate, const int support[4], unsigned int seed,
+                              float alpha, int supp_flag, const float *time_iteration,
+                              const float *time_supp_set, const int *cg_per_iteration,
+                              const float *time_for_cg) {
+  const int a = parse_alg(alg), e = parse_ens(ens);
+  if (a < 0 || e < 0) return set_err(XCS_EINVAL, "unknown algorithm or ensemble");
+  if (!norms || !times || !support) return set_err(XCS_EINVAL, "NULL argument");
+  XCS_TRY(timing_args_ok(a, iterations, time_iteration, time_supp_set, cg_per_iteration, time_for_cg));
+  std::string text;
+  XCS_TRY(results_timing_text(text, alg, a, e, k, m, n, p, matrix_ensemble, vec_distribution, norms,
+                              times, iterations, conv_rate, support, seed, alpha, supp_flag,
+                              time_iteration, time_supp_set, cg_per_iteration, time_for_cg));
+  // gpu_timings_data_<ens>YYYYMMDD.txt (results.cu:1540-1560 and twins)
+  const time_t now = time(nullptr);
+  struct tm tmv;
+  localtime_r(&now, &tmv);
+  char name[64];
+  snprintf(name, sizeof(name), "gpu_timings_data_%s%d%02d%02d.txt", ens, tmv.tm_year + 1900,
+           tmv.tm_mon + 1, tmv.tm_mday);
+  std::string path = (dir && *dir) ? std::string(dir) + "/" + name : std::string(name);
+  FILE *f = fopen(path.c_str(), "a+");
+  if (!f) return set_err(XCS_EINVAL, "cannot open %s for appending", path.c_str());
+  const bool ok = fputs(text.c_str(), f) >= 0;
+  if (fclose(f) != 0 || !ok) return set_err(XCS_EINVAL, "cannot write %s", path.c_str());
+  return XCS_OK;
+}
+
+int xcs_plan_times(xcs_plan *pl, float times[3]) {
+  if (!pl || !times) return set_err(XCS_EINVAL, "NULL argument");
+  // as xcs_cs_generate reports them: solve, solve per iteration, solve + generation (ms)
+  times[0] = pl->last_ms;
+  times[1] = pl->last.iter > 0 ? pl->last_ms / (float)pl->last.iter : 0.f;
+  times[2] = pl->last_ms + pl->gen_ms;
+  return XCS_OK;
+}
+
 int xcs_results_line(char *buf, size_t buf_len, const char *alg, const char *ens, int k, int m,
                      int n, int p, int matrix_ensemble, int vec_distribution, const float norms[3],
                      const float times[3], int iterations, float conv_rate, const int support[4],

@@ -98,6 +98,24 @@ int xcs_results_append(const char *dir, const char *alg, const char *ens, int k,
                        const float times[3], int iterations, float conv_rate,
                        const int support[4], unsigned int seed);
 
+/* The timing variants' results text (File_output_timings*, functions.cu:82-212): the line
+ * with alpha and supp_flag, then one line per iteration (HTP: with its CG steps and CG
+ * time; cg_per_iteration and time_for_cg may be NULL for NIHT).  Returns its length. */
+int xcs_results_timing_text(char *buf, size_t buf_len, const char *alg, const char *ens, int k,
+                            int m, int n, int p, int matrix_ensemble, int vec_distribution,
+                            const float norms[3], const float times[3], int iterations,
+                            float conv_rate, const int support[4], unsigned int seed, float alpha,
+                            int supp_flag, const float *time_iteration, const float *time_supp_set,
+                            const int *cg_per_iteration, const float *time_for_cg);
+/* Appends that text to <dir>/gpu_timings_data_<ens>YYYYMMDD.txt (results.cu:1540-1560). */
+int xcs_results_timing_append(const char *dir, const char *alg, const char *ens, int k, int m,
+                              int n, int p, int matrix_ensemble, int vec_distribution,
+                              const float norms[3], const float times[3], int iterations,
+                              float conv_rate, const int support[4], unsigned int seed,
+                              float alpha, int supp_flag, const float *time_iteration,
+                              const float *time_supp_set, const int *cg_per_iteration,
+                              const float *time_for_cg);
+
 /* Solve-given-problem forms ([outputVector iterations convRate] = gaga_cs(alg,
  * 'gen', y, A, k, options), gaga_gen.cu:115-134).  A is column-major m x n. */
 int xcs_cs_gen(const char *alg, const float *y, const float *A, int m, int n, int k,
@@ -119,6 +137,9 @@ typedef struct xcs_plan xcs_plan;
  * algorithms_timings.cu:1540-1543), for the first len iterations; any may be NULL. */
 int xcs_plan_timings(xcs_plan *plan, float *iter_ms, float *supp_ms, int *cg_iters, float *cg_ms,
                      int len);
+/* The times of the plan's last solve as xcs_cs_generate reports them (ms): solve, solve
+ * per iteration, solve + generation (the reference's timeALG, timeIter, timeTotal). */
+int xcs_plan_times(xcs_plan *plan, float times[3]);
 /* Generate one instance on the device (seed/instance from opts) and keep it. */
 int xcs_plan_create(const char *alg, const char *ens, int k, int m, int n, int p,
                     const xcs_options *opts, xcs_plan **plan);

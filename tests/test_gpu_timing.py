@@ -97,3 +97,25 @@ def test_batch_equals_plans():
         assert rb["iters"][i] == r["iters"]
         pl.close()
     B.close()
+
+
+def test_generate_writes_timing_results(tmp_path):
+    """The generate form with timing on appends the timing variants' text (results.cu:
+    1540-1560): the line, then one line per iteration with its measured times."""
+    import datetime
+    import gaga_amd as ga
+    norms, times, iters, sup, rate, x = ga.cs(
+        "HTP", "gen", 20, 256, 1024,
+        ga.options(tol=1e-5, seed=SEED, timing="on", supportFlag=2, resultsDir=str(tmp_path)))
+    assert norms[2] < 1e-3 and iters > 0
+    name = "gpu_timings_data_gen%s.txt" % datetime.date.today().strftime("%Y%m%d")
+    lines = open(tmp_path / name).read().splitlines()
+    assert lines[0].startswith("HTP_S_gen output: k 20, m 256, n 1024, ")
+    assert lines[0].endswith("alpha 2.5000000e-01 supp_flag 2 matrixEnsemble 1 ")
+    assert "iterations %d " % iters in lines[0]
+    assert len(lines) == 1 + iters
+    for j, line in enumerate(lines[1:]):
+        f = line.split()
+        assert f[0] == "iteration" and int(f[1]) == j
+        t_it, t_supp, cg, t_cg = float(f[3]), float(f[5]), float(f[7]), float(f[9])
+        assert 0 < t_supp <= t_it and 1 <= cg <= 15 and 0 < t_cg <= t_it

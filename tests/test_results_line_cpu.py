@@ -154,3 +154,68 @@ def test_smv_ensemble_names(lib):
         o = ga._apply_options("CGIHT", "smv", 1024, ga.options(matrixEnsemble=name))
         assert o.matrix_ensemble == code
     assert ga._apply_options("CGIHT", "smv", 1024, None).matrix_ensemble == 3
+
+
+# ----------------------------------------------------- timing variants' text --
+# File_output_timings_gen / _smv / File_output_timings (NIHT) and the _HTP twins
+# (functions.cu:82-212), applied by hand to exactly representable values.
+T_ITER = np.array([0.125, 0.25], np.float32)
+T_SUPP = np.array([0.0625, 0.03125], np.float32)
+CG = np.array([7, 15], np.int32)
+T_CG = np.array([0.5, 0.75], np.float32)
+HEAD = ("k 20, m 256, n 1024, vecDistribution {vec} "
+        "errorl1 1.2500000e-07 errorl2 2.5000000e-07 errorlinf 5.0000000e-07 "
+        "timeALG 1.5000000e+00 timeIter 1.0156250e-01 timeTotal 2.5000000e+00 "
+        "iterations 2 converge_rate 5.0000000e-01 "
+        "TruePos 20 FalsePos 0 TrueNeg 1004 FalseNeg 0 seed 7 alpha 2.5000000e-01 ")
+TIMING_GOLDEN = {
+    ("NIHT", "gen"): "NIHT_S_gen output: " + HEAD.format(vec=1) + "supp_flag 3 matrixEnsemble 1 \n"
+    "iteration 0 time_iteration 1.2500000e-01 time_supp_set 6.2500000e-02\n"
+    "iteration 1 time_iteration 2.5000000e-01 time_supp_set 3.1250000e-02\n",
+    ("HTP", "smv"): "HTP_S_smv output: " + HEAD.format(vec=2)
+    + "supp_flag 3 NonzerosPerCol 7 matrixEnsemble 3 \n"
+    "iteration 0 time_iteration 1.2500000e-01 time_supp_set 6.2500000e-02 cg_per_iteration 7 "
+    "time_for_cg 5.0000000e-01\n"
+    "iteration 1 time_iteration 2.5000000e-01 time_supp_set 3.1250000e-02 cg_per_iteration 15 "
+    "time_for_cg 7.5000000e-01\n",
+    ("NIHT", "dct"): "NIHT_S_dct output: " + HEAD.format(vec=0) + "supp_flag 3 \n"
+    "iteration 0 time_iteration 1.2500000e-01 time_supp_set 6.2500000e-02\n"
+    "iteration 1 time_iteration 2.5000000e-01 time_supp_set 3.1250000e-02\n",
+}
+
+
+@pytest.mark.parametrize("alg,ens", list(TIMING_GOLDEN))
+def test_timing_text_golden(alg, ens, lib):
+    import gaga_amd as ga
+    o = ga._capi.default_options(alg, ens, 1024)
+    o.vec_distribution = VEC[ens]
+    o.matrix_ensemble = MAT[ens]
+    o.seed = 7
+    htp = alg == "HTP"
+    text = ga._capi.results_timing_text(alg, ens, 20, 256, 1024, 7, o, NORMS, TIMES, 2, SUP, 0.5,
+                                        0.25, 3, T_ITER, T_SUPP, CG if htp else None,
+                                        T_CG if htp else None)
+    assert text == TIMING_GOLDEN[(alg, ens)]
+
+
+def test_timing_append_file(tmp_path, lib):
+    import gaga_amd as ga
+    o = ga._capi.default_options("NIHT", "gen", 1024)
+    o.vec_distribution, o.matrix_ensemble, o.seed = 1, 1, 7
+    for _ in range(2):
+        ga._capi.results_timing_append(tmp_path, "NIHT", "gen", 20, 256, 1024, 0, o, NORMS, TIMES,
+                                       2, SUP, 0.5, 0.25, 3, T_ITER, T_SUPP)
+    name = "gpu_timings_data_gen%s.txt" % datetime.date.today().strftime("%Y%m%d")
+    assert os.listdir(tmp_path) == [name]
+    assert open(tmp_path / name).read() == TIMING_GOLDEN[("NIHT", "gen")] * 2
+
+
+def test_timing_text_errors(lib):
+    import gaga_amd as ga
+    o = ga._capi.default_options("NIHT", "gen", 1024)
+    with pytest.raises(ga.GagaError):  # only NIHT and HTP have timing variants
+        ga._capi.results_timing_text("CGIHT", "gen", 20, 256, 1024, 0, o, NORMS, TIMES, 2, SUP,
+                                     0.5, 0.25, 0, T_ITER, T_SUPP)
+    with pytest.raises(ga.GagaError):  # HTP needs its CG arrays
+        ga._capi.results_timing_text("HTP", "gen", 20, 256, 1024, 0, o, NORMS, TIMES, 2, SUP,
+                                     0.5, 0.25, 0, T_ITER, T_SUPP)
