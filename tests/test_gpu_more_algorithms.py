@@ -118,3 +118,30 @@ def test_cgihtp_batch_bit_identical_to_single_instance():
         assert rb["norms"][i, 2] < 1e-3
         pl.close()
     B.close()
+
+
+@pytest.mark.parametrize("alg,sf", [(a, None) for a in ALGS] + [("NIHT", s) for s in range(4)]
+                         + [("HTP", s) for s in range(4)])
+def test_edge_cases(alg, sf):
+    """gen 50 x 77 (neither a multiple of 64): y = 0 returns x = 0 without iterating,
+    k = 0 returns x = 0 (the support stays empty), k = 3 recovers a 3-sparse x; the
+    sort-based supportFlags refuse k = 0."""
+    import gaga_amd as ga
+    m, n = 50, 77
+    rng = np.random.default_rng(0)
+    A = (rng.standard_normal((m, n)) / np.sqrt(m)).astype(np.float32)
+    extra = {} if sf is None else dict(timing="on", supportFlag=sf)
+    print("case edge", alg, extra, flush=True)
+    x, it, _ = ga.cs(alg, "gen", np.zeros(m, np.float32), A, 3, ga.options(tol=1e-5, **extra))
+    assert it == 0 and not x.any()
+    xt = np.zeros(n, np.float32)
+    xt[:3] = 1
+    y = (A @ xt).astype(np.float32)
+    if sf is not None and sf >= 2:
+        with pytest.raises(ga.GagaError, match="k >= 1"):
+            ga.cs(alg, "gen", y, A, 0, ga.options(tol=1e-5, maxiter=20, **extra))
+    else:
+        x, it, _ = ga.cs(alg, "gen", y, A, 0, ga.options(tol=1e-5, maxiter=20, **extra))
+        assert not x.any()
+    x, it, _ = ga.cs(alg, "gen", y, A, 3, ga.options(tol=1e-5, maxiter=200, **extra))
+    assert np.max(np.abs(x - xt)) < 1e-3
