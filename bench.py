@@ -258,6 +258,8 @@ def main():
     ap.add_argument("--batch", type=int, default=640, help="instances per step")
     ap.add_argument("--block", type=int, default=1024)
     ap.add_argument("--team", type=int, default=1)
+    ap.add_argument("--big", type=int, default=-1,
+                    help="instances solved on teams of --team workgroups (-1: all when team > 1)")
     ap.add_argument("--plain-instances", type=int, default=8)
     ap.add_argument("--cpu-sample", type=int, default=400, help="most CPU-baseline instances")
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
@@ -287,6 +289,8 @@ def main():
     specs = [dict(ens="gen", k=K, m=M, n=N, instance=i) for i in range(args.batch)]
     log("generating %d instances of %dx%d ..." % (args.batch, M, N))
     B = ga.Batch("CGIHT", specs, opts)
+    if args.big >= 0:
+        B.configure(args.team, args.block, args.big)
     for _ in range(args.warmup):
         B.solve()
     sync()
