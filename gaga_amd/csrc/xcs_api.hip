@@ -510,7 +510,7 @@ static int plan_alloc(xcs_plan *pl, int alg, int ens, int k, int m, int n, int p
   XCS_TRY(pl->alloc(&P.counters, (size_t)P.num_bins + 64));
   XCS_TRY(pl->alloc(&P.supp, nn));
   XCS_TRY(pl->alloc(&P.wg_counts, (size_t)pl->team + 64));
-  if (ens == ENS_GEN) XCS_TRY(pl->alloc(&P.partials, (size_t)MAXCH * ml));
+  if (ens == ENS_GEN) XCS_TRY(pl->alloc(&P.partials, 2 * (size_t)MAXCH * ml));
   if (ens == ENS_DCT) {
     XCS_TRY(pl->alloc(&P.cwork, 3 * nn));
     float2 *tw;
@@ -913,7 +913,7 @@ static int batch_configure(xcs_batch *B, int c, int block, int big = -1) {
   XCS_TRY(B->walloc(&f_m, T * 4 * ml));
   XCS_TRY(B->walloc(&cnt, T * ((size_t)B->bins_max + 64)));
   XCS_TRY(B->walloc(&wgc, T * ((size_t)c + 64)));
-  XCS_TRY(B->walloc(&part, T * MAXCH * ml));
+  XCS_TRY(B->walloc(&part, T * 2 * MAXCH * ml));
   if (B->any_dct) XCS_TRY(B->walloc(&cw, T * 3 * nn));
   std::vector<XWork> hw(T);
   for (size_t t = 0; t < T; t++) {
@@ -931,7 +931,7 @@ static int batch_configure(xcs_batch *B, int c, int block, int big = -1) {
     xset(W.resid3, f_m + (t * 4 + 3) * ml);
     xset(W.counters, cnt + t * ((size_t)B->bins_max + 64));
     xset(W.wg_counts, wgc + t * ((size_t)c + 64));
-    xset(W.partials, part + t * MAXCH * ml);
+    xset(W.partials, part + t * 2 * MAXCH * ml);
     xset(W.cwork, cw ? cw + t * 3 * nn : nullptr);
   }
   XCS_TRY(B->walloc(&B->dW, T));

@@ -65,7 +65,7 @@ struct XProblem {
   XG(int) counters;           // >= num_bins
   XG(int) supp;               // n: ascending support list
   XG(int) wg_counts;          // team size
-  XG(float) partials;         // MAXCH * lda (gen gather partials)
+  XG(float) partials;         // 2 * MAXCH * lda (gen gather partials, two outputs)
   XG(float2) cwork;           // dct: 3 * n complex scratch
   XG(const float2) twiddle;   // dct: n roots of unity, then n quarter-wave
   XG(XState) st;

@@ -29,6 +29,7 @@ union Lds {
   struct {
     int j[SLDS];
     float v[SLDS];
+    float w[SLDS];  // the second vector of a two-output gather
   } g;
 };
 

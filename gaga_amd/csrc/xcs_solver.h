@@ -50,9 +50,12 @@ namespace xcs {
 // The support indices and values are staged in LDS so every column load is
 // independent; work items are (256-row tile, chunk of cs support columns);
 // chunk partials are summed in a fixed order.  out is written with strided
-// ownership.  Reads v: caller makes it visible.
-__device__ void gen_A(Team &t, Lds &L, const XProblem &P, XG(const float) v, XG(float) out,
-                      int cnt) {
+// ownership.  Reads v: caller makes it visible.  NV = 2 forms A v0 and A v1
+// over the same support in one pass over its columns (each output with the
+// arithmetic of its own NV = 1 pass: same chunks, same order).
+template <int NV>
+__device__ void gen_A_multi(Team &t, Lds &L, const XProblem &P, XG(const float) v0,
+                            XG(float) out0, XG(const float) v1, XG(float) out1, int cnt) {
   const int lane = threadIdx.x & (WAVE - 1);
   const int lda = P.lda, nq = lda >> 2;
   XG(const float) A = P.A;
@@ -78,7 +81,8 @@ __device__ void gen_A(Team &t, Lds &L, const XProblem &P, XG(const float) v, XG(
     for (int i = threadIdx.x; i < s_cnt; i += blockDim.x) {
       const int j = supp[s_base + i];
       L.g.j[i] = j;
-      L.g.v[i] = v[j];
+      L.g.v[i] = v0[j];
+      if (NV == 2) L.g.w[i] = v1[j];
     }
     __syncthreads();
     PHASE(P, t, 18);
@@ -88,15 +92,17 @@ __device__ void gen_A(Team &t, Lds &L, const XProblem &P, XG(const float) v, XG(
       const int q = rt * WAVE + lane;
       if (q < nq) {
         float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+        float4 acc2 = make_float4(0.f, 0.f, 0.f, 0.f);
         const int s0 = ch * cs, s1 = min(s_cnt, s0 + cs);
         int s = s0;
         for (; s + 4 <= s1; s += 4) {
           float4 a[4];
-          float xv[4];
+          float xv[4], xw[4];
 #pragma unroll
           for (int u = 0; u < 4; u++) {
             a[u] = reinterpret_cast<XG(const float4)>(A + (size_t)L.g.j[s + u] * lda)[q];
             xv[u] = L.g.v[s + u];
+            if (NV == 2) xw[u] = L.g.w[s + u];
           }
 #pragma unroll
           for (int u = 0; u < 4; u++) {
@@ -104,6 +110,12 @@ __device__ void gen_A(Team &t, Lds &L, const XProblem &P, XG(const float) v, XG(
             acc.y = fmaf(a[u].y, xv[u], acc.y);
             acc.z = fmaf(a[u].z, xv[u], acc.z);
             acc.w = fmaf(a[u].w, xv[u], acc.w);
+            if (NV == 2) {
+              acc2.x = fmaf(a[u].x, xw[u], acc2.x);
+              acc2.y = fmaf(a[u].y, xw[u], acc2.y);
+              acc2.z = fmaf(a[u].z, xw[u], acc2.z);
+              acc2.w = fmaf(a[u].w, xw[u], acc2.w);
+            }
           }
         }
         for (; s < s1; s++) {
@@ -113,8 +125,16 @@ __device__ void gen_A(Team &t, Lds &L, const XProblem &P, XG(const float) v, XG(
           acc.y = fmaf(a.y, xv, acc.y);
           acc.z = fmaf(a.z, xv, acc.z);
           acc.w = fmaf(a.w, xv, acc.w);
+          if (NV == 2) {
+            const float xw = L.g.w[s];
+            acc2.x = fmaf(a.x, xw, acc2.x);
+            acc2.y = fmaf(a.y, xw, acc2.y);
+            acc2.z = fmaf(a.z, xw, acc2.z);
+            acc2.w = fmaf(a.w, xw, acc2.w);
+          }
         }
         reinterpret_cast<XG(float4)>(part + (size_t)(ch0 + ch) * lda)[q] = acc;
+        if (NV == 2) reinterpret_cast<XG(float4)>(part + (size_t)(MAXCH + ch0 + ch) * lda)[q] = acc2;
       }
     }
     PHASE(P, t, 19);
@@ -125,9 +145,18 @@ __device__ void gen_A(Team &t, Lds &L, const XProblem &P, XG(const float) v, XG(
   for (int i = t.gtid(); i < P.m; i += t.gsize()) {
     float s = 0.f;
     for (int c = 0; c < nch; c++) s += part[(size_t)c * lda + i];
-    out[i] = s;
+    out0[i] = s;
+    if (NV == 2) {
+      float s2 = 0.f;
+      for (int c = 0; c < nch; c++) s2 += part[(size_t)(MAXCH + c) * lda + i];
+      out1[i] = s2;
+    }
   }
   PHASE(P, t, 21);
+}
+__device__ void gen_A(Team &t, Lds &L, const XProblem &P, XG(const float) v, XG(float) out,
+                      int cnt) {
+  gen_A_multi<1>(t, L, P, v, out, nullptr, nullptr, cnt);
 }
 
 // g = A^T r, dense (gen_matvec on the transpose, gen_gpu.cu:33-74, 161-171),
@@ -629,8 +658,15 @@ __device__ float step_cg_restricted(Team &t, Lds &L, const XProblem &P, XState &
 __device__ float step_cg_unrestricted(Team &t, Lds &L, const XProblem &P, XState &S, float *xmax) {
   const int kb = S.k_bin;
   PHASE_BEGIN();
-  // resid_update = -(A x - y)
-  opA(t, L, P, S, P.x, P.resid_update);
+  // resid_update = -(A x - y).  gen: A x and A p_old_S (the previous direction on the
+  // current support, final at this point) share one pass over the support's columns
+  const bool fused = P.ens == ENS_GEN;
+  if (fused) {
+    S.mat_bytes += op_bytes(P, false, S.supp_count) + 8.0 * S.supp_count + 4.0 * P.m;
+    gen_A_multi<2>(t, L, P, P.x, P.resid_update, P.gprev_thres, P.resid, S.supp_count);
+  } else {
+    opA(t, L, P, S, P.x, P.resid_update);
+  }
   PHASE(P, t, 15);
   double e2 = 0.0;
   for (int i = t.gtid(); i < P.m; i += t.gsize()) {
@@ -651,7 +687,7 @@ __device__ float step_cg_unrestricted(Team &t, Lds &L, const XProblem &P, XState
   });
   t.sync();
   PHASE(P, t, 7);
-  opA(t, L, P, S, P.gprev_thres, P.resid);
+  if (!fused) opA(t, L, P, S, P.gprev_thres, P.resid);
   t.sync();
   PHASE(P, t, 8);
   opA(t, L, P, S, P.thres, P.resid_update);
