@@ -370,8 +370,7 @@ static int device_smv(hipStream_t st, int m, int n, int p, int ens, uint32_t see
   HIP_TRY(hipMemsetAsync(first, 0, sizeof(int) * (size_t)m, st));
   hipLaunchKernelGGL(smv_fill_kernel, dim3((n + 256) / 256), dim3(256), 0, st, m, n, p, d.rows,
                      d.vals, d.row_ptr, first, d.col_ptr, d.cols, d.rvals);
-  hipLaunchKernelGGL(smv_row_sort_kernel, dim3((m + 255) / 256), dim3(256), 0, st, m, d.row_ptr,
-                     d.cols, d.rvals);
+  hipLaunchKernelGGL(smv_row_sort_kernel, dim3(m), dim3(64), 0, st, m, d.row_ptr, d.cols, d.rvals);
   HIP_TRY(hipGetLastError());
   return XCS_OK;
 }

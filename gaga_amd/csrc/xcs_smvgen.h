@@ -164,22 +164,45 @@ __global__ void smv_fill_kernel(int m, int n, int p, const int *rows, const floa
   }
 }
 
-// Columns ascending within each row (one thread per row, insertion sort).
-__global__ void smv_row_sort_kernel(int m, const int *row_ptr, int *cols, float *rvals) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+// Columns ascending within each row.  The columns of a row are distinct, so an
+// entry's place is the number of smaller columns in its row: one 64-thread
+// workgroup per row stages the row in LDS, ranks every entry against it and
+// writes it to its place.  Rows longer than SMV_SORT_LDS (m tiny against n p)
+// fall back to an insertion sort by one thread.
+constexpr int SMV_SORT_LDS = 4096;
+__global__ void __launch_bounds__(64) smv_row_sort_kernel(int m, const int *row_ptr, int *cols,
+                                                          float *rvals) {
+  __shared__ int sc[SMV_SORT_LDS];
+  __shared__ float sv[SMV_SORT_LDS];
+  const int r = blockIdx.x;
   if (r >= m) return;
-  const int lo = row_ptr[r], hi = row_ptr[r + 1];
-  for (int a = lo + 1; a < hi; a++) {
-    const int c = cols[a];
-    const float v = rvals[a];
-    int b = a - 1;
-    while (b >= lo && cols[b] > c) {
-      cols[b + 1] = cols[b];
-      rvals[b + 1] = rvals[b];
-      b--;
+  const int lo = row_ptr[r], len = row_ptr[r + 1] - lo;
+  if (len <= SMV_SORT_LDS) {
+    for (int i = threadIdx.x; i < len; i += blockDim.x) {
+      sc[i] = cols[lo + i];
+      sv[i] = rvals[lo + i];
     }
-    cols[b + 1] = c;
-    rvals[b + 1] = v;
+    __syncthreads();
+    for (int i = threadIdx.x; i < len; i += blockDim.x) {
+      const int c = sc[i];
+      int rank = 0;
+      for (int q = 0; q < len; q++) rank += sc[q] < c ? 1 : 0;
+      cols[lo + rank] = c;
+      rvals[lo + rank] = sv[i];
+    }
+  } else if (threadIdx.x == 0) {
+    for (int a = lo + 1; a < lo + len; a++) {
+      const int c = cols[a];
+      const float v = rvals[a];
+      int b = a - 1;
+      while (b >= lo && cols[b] > c) {
+        cols[b + 1] = cols[b];
+        rvals[b + 1] = rvals[b];
+        b--;
+      }
+      cols[b + 1] = c;
+      rvals[b + 1] = v;
+    }
   }
 }
 

@@ -30,10 +30,11 @@ def test_gen_matrix_matches_oracle(oracle, lib):
 @pytest.mark.parametrize("m,n,p,ens,inst", [(1000, 1024, 2, 2, 3), (60, 64, 1, 2, 1),
                                             (900, 1024, 2, 3, 2), (256, 2048, 7, 2, 1),
                                             (3000, 3000, 7, 3, 9), (64, 64, 64, 2, 1),
-                                            (32768, 262144, 7, 3, 0)])
+                                            (8, 16384, 4, 2, 1), (32768, 262144, 7, 3, 0)])
 def test_smv_generator_matches_oracle(m, n, p, ens, inst, oracle, lib):
     """Device smv draws, zero-row repair and CSR copy equal the oracle bit for bit
-    (the first five cases have empty rows before the repair)."""
+    (the first five cases have empty rows before the repair; m = 8, n = 16384 has
+    rows longer than the row sort stages in LDS)."""
     from gaga_amd import _capi
     print("case smv generator", m, n, p, ens, inst, flush=True)
     rows, vals, rp, cols, rv, empty = _capi.generate_smv(m, n, p, ens, SEED, inst)
