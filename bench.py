@@ -61,7 +61,7 @@ def measured_traffic(args):
     """HBM bytes per launch of the solve kernel from the committed rocprofv3
     FETCH_SIZE/WRITE_SIZE runs (copy-kernel calibrated), for the profiled
     configuration only; None otherwise."""
-    if (args.batch, args.block, args.team) != (640, 1024, 1):
+    if (args.batch, args.block, args.team, args.big) != (640, 1024, 1, -1):
         return None
     path = os.path.join(ROOT, "profiles", "traffic_solve_batch_kernel.txt")
     try:

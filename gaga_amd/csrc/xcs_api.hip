@@ -352,7 +352,8 @@ struct SmvOut {
   int *col_ptr, *rows, *row_ptr, *cols;
   float *vals, *rvals;
 };
-static int smv_check(int m, int n, int p) {
+static int smv_check(int m, int n, int p, int ens) {
+  if (ens < 1 || ens > 3) return set_err(XCS_EINVAL, "smv: matrixEnsemble must be 1, 2 or 3");
   if (p <= 0 || p > m) return set_err(XCS_EINVAL, "smv: need 0 < p <= m (p = %d, m = %d)", p, m);
   if (p > SMV_P_MAX) return set_err(XCS_EINVAL, "smv: the generator supports p <= %d", SMV_P_MAX);
   if ((long)n * p >= (long)INT_MAX) return set_err(XCS_EINVAL, "smv: n * p too large");
@@ -626,7 +627,7 @@ static int plan_generate(xcs_plan *pl, int alg, int ens, int k, int m, int n, in
   int *smv_tmp = nullptr;
   int nz = 0;
   if (ens == ENS_SMV) {
-    XCS_TRY(smv_check(m, n, p));
+    XCS_TRY(smv_check(m, n, p, o->matrix_ensemble));
     nz = n * p;
   }
   XCS_TRY(plan_alloc(pl, alg, ens, k, m, n, p, nz, o));
@@ -956,7 +957,7 @@ static int batch_generate(xcs_batch *B, int alg, int count, const int *ens, cons
     if (k[i] > n[i]) return set_err(XCS_EINVAL, "instance %d: k > n", i);
     if (ens[i] == ENS_SMV) {
       if (!p) return set_err(XCS_EINVAL, "instance %d: smv needs p", i);
-      XCS_TRY(smv_check(m[i], n[i], p[i]));
+      XCS_TRY(smv_check(m[i], n[i], p[i], o->matrix_ensemble));
     }
   }
   XCS_TRY(use_device(o->gpu_number));
@@ -1922,7 +1923,7 @@ int xcs_plan_timings(xcs_plan *pl, float *iter_ms, float *supp_ms, int *cg_iters
 int xcs_generate_smv(int m, int n, int p, int ens, unsigned int seed, unsigned int instance,
                      int *rows, float *vals, int *row_ptr, int *cols, float *rvals, int *empty) {
   if (!rows || !vals) return set_err(XCS_EINVAL, "NULL argument");
-  XCS_TRY(smv_check(m, n, p));
+  XCS_TRY(smv_check(m, n, p, ens));
   XCS_TRY(use_device(0));
   xcs_plan pl;
   HIP_TRY(hipStreamCreateWithFlags(&pl.stream, hipStreamNonBlocking));
