@@ -119,3 +119,22 @@ def test_c2_gen_more_algorithms(alg, oracle):
             np.testing.assert_array_equal(np.abs(xd) > big, np.abs(x64) > big)
             assert np.linalg.norm(xd - x64) <= 1e-3 * np.linalg.norm(x64)
             assert abs(it - r64.iters) <= max(2, r64.iters // 10)
+
+
+@pytest.mark.parametrize("alg", ["FIHT", "ALPS", "CGIHTprojected"])
+@pytest.mark.parametrize("ens,k,m,n,p", [("smv", 3277, 32768, 262144, 7),
+                                         ("dct", 13108, 131072, 1048576, 0)])
+def test_c2_full_size_more_algorithms(alg, ens, k, m, n, p):
+    """smv 2^18 and dct 2^20 at delta = 1/8, rho = 1/10, generated on the device: FIHT,
+    ALPS and CGIHTprojected recover the planted signal on all four instances (relative
+    l_inf error < 1e-3, the reference's success test)."""
+    import gaga_amd as ga
+    for inst in range(4):
+        print("case C2", ens, alg, "instance", inst, flush=True)
+        extra = {"matrixEnsemble": "binary"} if ens == "smv" else {}
+        pl = ga.Plan(alg, ens, k, m, n, p, ga.options(tol=1e-5, seed=SEED, vecDistribution="gaussian",
+                                                      instance=inst, **extra))
+        pl.solve()
+        r = pl.results()
+        assert r["norms"][2] < 1e-3, (inst, r["iters"], r["stop_reason"], r["norms"])
+        pl.close()
