@@ -494,9 +494,15 @@ __device__ float step_restricted_sd(Team &t, Lds &L, const XProblem &P, XState &
 // RestrictedCG_S_* (functions.cu:752-1000): one CG step on the support.
 // grad = search direction, gprev = CG residual, S.mu carries <r,r>.
 // Pushes the squared residual onto hist.
-__device__ void step_restricted_cg(Team &t, Lds &L, const XProblem &P, XState &S, float *hist) {
+__device__ void step_restricted_cg(Team &t, Lds &L, const XProblem &P, XState &S, float *hist,
+                                   bool ad_ready = false, bool fuse_next = false) {
   const int kb = S.k_bin;
-  opA(t, L, P, S, P.grad, P.resid);
+  if (ad_ready) {
+    // A d was formed by the previous step's two-output pass (into resid3)
+    for (int i = t.gtid(); i < P.m; i += t.gsize()) P.resid[i] = P.resid3[i];
+  } else {
+    opA(t, L, P, S, P.grad, P.resid);
+  }
   float a2 = 0.f;
   for (int i = t.gtid(); i < P.m; i += t.gsize()) a2 = fmaf(P.resid[i], P.resid[i], a2);
   float alpha;
@@ -532,7 +538,21 @@ __device__ void step_restricted_cg(Team &t, Lds &L, const XProblem &P, XState &S
   for (int i = t.gtid(); i < P.n; i += t.gsize())
     P.grad[i] = __fadd_rn(__fmul_rn(beta, P.grad[i]), P.gprev[i]);
   t.sync();
-  double e2 = residual(t, L, P, S, P.resid, P.resid_update);
+  double e2;
+  if (fuse_next) {
+    // gen: A x for the residual and A d for the next step in one pass over the
+    // support's columns (each output as its own pass would form it)
+    S.mat_bytes += op_bytes(P, false, S.supp_count) + 8.0 * S.supp_count + 4.0 * P.m;
+    gen_A_multi<2>(t, L, P, P.x, P.resid_update, P.grad, P.resid3, S.supp_count);
+    e2 = 0.0;
+    for (int i = t.gtid(); i < P.m; i += t.gsize()) {
+      const float r = __fsub_rn(P.y[i], P.resid_update[i]);
+      P.resid[i] = r;
+      e2 += (double)r * (double)r;
+    }
+  } else {
+    e2 = residual(t, L, P, S, P.resid, P.resid_update);
+  }
   double v[1] = {e2};
   const int op[1] = {0};
   reduce(t, v, op);
@@ -570,8 +590,10 @@ __device__ float cg_projection(Team &t, Lds &L, const XProblem &P, XState &S, bo
   float errCG = (float)v[1];
   hist[15] = errCG;
   int iterCG = 0;
+  const bool gen = P.ens == ENS_GEN;
   while ((errCG > tolCG) & (iterCG < max_steps) & (diffCG > tol2)) {
-    step_restricted_cg(t, L, P, S, hist);
+    // gen: each step but the last allowed one also forms the next step's A d
+    step_restricted_cg(t, L, P, S, hist, gen && iterCG > 0, gen && iterCG + 1 < max_steps);
     errCG = hist[15];
     iterCG++;
 #pragma unroll
