@@ -10,6 +10,7 @@
 #include <cstring>
 #include <ctime>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/gaga_amd.h"
@@ -779,6 +780,29 @@ static void host_norms(const std::vector<float> &xt, const float *xo, int k, flo
   }
 }
 
+// f(0), ..., f(count - 1), each once, over up to 16 host threads (fewer for
+// small counts); a thread that cannot be started has its share run here.
+template <class F>
+static void host_parallel_for(int count, F f) {
+  const long hw = std::max(1u, std::thread::hardware_concurrency());
+  const int nt = (int)std::max(1L, std::min({16L, hw, (long)(count + 31) / 32}));
+  auto share = [&](int w) {
+    for (int i = w; i < count; i += nt) f(i);
+  };
+  std::vector<std::thread> th;
+  std::vector<int> here;
+  for (int w = 1; w < nt; w++) {
+    try {
+      th.emplace_back(share, w);
+    } catch (...) {
+      here.push_back(w);
+    }
+  }
+  share(0);
+  for (int w : here) share(w);
+  for (auto &t : th) t.join();
+}
+
 // ----------------------------------------------------------------- batches --
 // Instance data lives in two arenas: one for the dense gen matrices (written
 // in full by the generator, never cleared) and one for everything else
@@ -795,6 +819,7 @@ struct xcs_batch {
   XProblem *dP = nullptr;
   float *xs = nullptr;       // every x, contiguous (queue order)
   size_t xs_len = 0;
+  float *xs_host = nullptr;  // pinned copy of xs for the results (allocated on first use)
   std::vector<size_t> xoff;
   XState *sts = nullptr;
   // team workspace (depends on c and block)
@@ -824,6 +849,7 @@ struct xcs_batch {
     drop_graph();
     drop_work();
     for (void *q : allocs) (void)hipFree(q);
+    if (xs_host) (void)hipHostFree(xs_host);
     if (stream) (void)hipStreamDestroy(stream);
   }
   int raw(void **q, size_t bytes, bool clear, std::vector<void *> &list) {
@@ -1976,14 +2002,25 @@ int xcs_batch_results(xcs_batch *B, float *norms, int *iterations, int *stop_rea
                       float *conv_rate) {
   if (!B) return set_err(XCS_EINVAL, "NULL batch");
   HIP_TRY(hipSetDevice(B->dev));
-  std::vector<float> xo;
   if (norms) {
-    xo.resize(B->xs_len);
-    HIP_TRY(hipMemcpy(xo.data(), B->xs, sizeof(float) * B->xs_len, hipMemcpyDeviceToHost));
+    if (!B->xs_host) {
+      const hipError_t e = hipHostMalloc((void **)&B->xs_host, sizeof(float) * B->xs_len, hipHostMallocDefault);
+      if (e != hipSuccess) {
+        B->xs_host = nullptr;
+        return set_err(XCS_ENOMEM, "hipHostMalloc(%zu bytes): %s", sizeof(float) * B->xs_len,
+                       hipGetErrorString(e));
+      }
+    }
+    HIP_TRY(hipMemcpy(B->xs_host, B->xs, sizeof(float) * B->xs_len, hipMemcpyDeviceToHost));
+    // instances are independent and each is one sequential loop, so the split
+    // over threads does not change any value
+    const float *xo = B->xs_host;
+    host_parallel_for(B->count, [&](int q) {
+      host_norms(B->x_true[q], xo + B->xoff[q], B->ks[q], norms + 3 * (size_t)B->order[q], nullptr);
+    });
   }
   for (int q = 0; q < B->count; q++) {
     const int i = B->order[q];
-    if (norms) host_norms(B->x_true[q], xo.data() + B->xoff[q], B->ks[q], norms + 3 * (size_t)i, nullptr);
     if (iterations) iterations[i] = B->last[q].iter;
     if (stop_reason) stop_reason[i] = B->last[q].stop_reason;
     if (conv_rate) conv_rate[i] = conv_rate_results(B->last[q]);
