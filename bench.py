@@ -3,7 +3,7 @@
 Gaussian nonzeros, FP32), one MI355X.
 
 A step is one pass of the hot path over one batch of synthetic input: one
-persistent launch that solves `--batch` resident instances (default 640, 40 GB
+persistent launch that solves `--batch` resident instances (default 640, 40 GiB
 of matrices; 256 workgroups of 1024 threads take them from a device queue)
 from x = 0 to the stopping rule.  value = recovered instances
 (relative l_inf error < 1e-3, GAGA/CGIHTcode/shared/tests_delta_fixed.m:17-23)
