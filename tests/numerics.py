@@ -19,3 +19,12 @@ def e4m3_rne(v):
     q = 2.0 ** (e - 3)
     r = np.round(a / q) * q  # numpy rounds half to even
     return np.sign(v) * np.minimum(r, 448.0)
+
+
+def rel_errors(x, x_true):
+    """Relative l1, l2 and l_inf errors of x against x_true in float64, the
+    closed forms of the reference's results (results.cu:363-378)."""
+    t = np.asarray(x_true, np.float64)
+    d = t - np.asarray(x, np.float64)
+    return np.array([np.abs(d).sum() / np.abs(t).sum(), np.linalg.norm(d) / np.linalg.norm(t),
+                     np.abs(d).max() / np.abs(t).max()])

@@ -13,6 +13,8 @@ reference's bins keep many more entries, with tiny values, so FalseNeg is not 0)
 import numpy as np
 import pytest
 
+from numerics import rel_errors
+
 pytestmark = pytest.mark.gpu
 
 SEED = 20260101
@@ -28,6 +30,11 @@ def test_c1_niht_256_seeds(oracle):
     ok = r["norms"][:, 2] < 1e-3
     print("C1: %d / 256 recovered, iterations median %d" % (ok.sum(), np.median(r["iters"])))
     assert ok.mean() >= 0.95
+    # every instance's reported errors are those of its own returned vector (the
+    # results are computed on several host threads)
+    for i in range(256):
+        x, xt = B.x(i)
+        np.testing.assert_allclose(r["norms"][i], rel_errors(x, xt), rtol=1e-6, atol=0)
     for i in range(4):
         x, xt = B.x(i)
         P = oracle.make_problem("gen", m, n, SEED, i)

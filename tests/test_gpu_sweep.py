@@ -3,6 +3,8 @@ graph replays, team geometries, the autotuner and its cache."""
 import numpy as np
 import pytest
 
+from numerics import rel_errors
+
 pytestmark = pytest.mark.gpu
 
 SEED = 20260101
@@ -44,6 +46,11 @@ def test_grid_matches_single_instances_and_replays():
                 np.testing.assert_array_equal(runs[0][1][i], rp["x"])
                 assert r["iters"][i] == rp["iters"]
                 pl.close()
+    # each reported error belongs to the caller's instance (the queue runs them
+    # largest first)
+    for i in range(len(specs)):
+        np.testing.assert_allclose(runs[0][0]["norms"][i], rel_errors(runs[0][1][i], B.x(i)[1]),
+                                   rtol=1e-6, atol=0)
     for r, xs in runs[1:]:
         np.testing.assert_array_equal(r["iters"], runs[0][0]["iters"])
         for a, b in zip(xs, runs[0][1]):

@@ -38,3 +38,11 @@ def test_e4m3_known_values():
     r = e4m3_rne(xs)
     assert (np.diff(r) >= 0).all()
     np.testing.assert_array_equal(e4m3_rne(r), r)
+
+
+def test_rel_errors_closed_form():
+    from numerics import rel_errors
+    t = np.array([1.0, -2.0, 0.0, 3.0], np.float32)
+    x = np.array([1.5, -2.0, 0.0, 2.0], np.float32)
+    np.testing.assert_allclose(rel_errors(x, t), [1.5 / 6, np.sqrt(1.25 / 14), 1 / 3], rtol=1e-15)
+    np.testing.assert_array_equal(rel_errors(t, t), [0.0, 0.0, 0.0])
